@@ -1,0 +1,329 @@
+"""Python binding of the MI355X sort-path C ABI (``include/gpusort_hip.h``).
+
+The product is the gfx950 shared library ``libgpusort_hip.so``; this module
+only loads it with ctypes and passes device pointers (torch tensors) or host
+buffers (numpy arrays) through.  There is no CPU fallback: if the library is
+missing, every call raises ``GpusortError``.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+from pathlib import Path
+
+import numpy as np
+
+_HERE = Path(__file__).resolve().parent
+LIB_PATH = _HERE / "libgpusort_hip.so"
+
+# gs_dtype codes (include/gpusort_hip.h)
+I32, U32, F32, I64, U64, F64, NONE = 0, 1, 2, 3, 4, 5, 255
+HYKSORT, SAMPLESORT, HYPERQUICKSORT = 0, 1, 2
+
+STATUS_NAMES = {
+    0: "GS_OK",
+    1: "GS_INVALID_ARG",
+    2: "GS_OUT_OF_MEMORY",
+    3: "GS_ADMISSION_REFUSED",
+    4: "GS_DEVICE_ERROR",
+    5: "GS_NO_DEVICE",
+    6: "GS_UNSUPPORTED",
+    7: "GS_EMPTY_INPUT",
+}
+
+_NP_DTYPE = {
+    np.dtype(np.int32): I32,
+    np.dtype(np.uint32): U32,
+    np.dtype(np.float32): F32,
+    np.dtype(np.int64): I64,
+    np.dtype(np.uint64): U64,
+    np.dtype(np.float64): F64,
+}
+_CODE_NP = {v: k for k, v in _NP_DTYPE.items()}
+
+
+class GpusortError(RuntimeError):
+    def __init__(self, status: int, message: str):
+        super().__init__(f"{STATUS_NAMES.get(status, status)}: {message}")
+        self.status = status
+        self.message = message
+
+
+class VrankReport(ctypes.Structure):
+    _fields_ = [
+        ("rounds", ctypes.c_uint32),
+        ("max_splitters", ctypes.c_uint32),
+        ("n_splitters", ctypes.c_uint32),
+        ("splitters", ctypes.c_void_p),
+        ("bucket_sizes", ctypes.POINTER(ctypes.c_uint64)),
+        ("max_bucket_sizes", ctypes.c_uint32),
+        ("n_bucket_sizes", ctypes.c_uint32),
+        ("final_rank_sizes", ctypes.POINTER(ctypes.c_uint64)),
+        ("max_rank_error", ctypes.c_double),
+    ]
+
+
+_lib = None
+
+# Exported C symbols and their argument types.
+_SIGS = {
+    "gs_version": ([], ctypes.c_char_p),
+    "gs_last_error": ([], ctypes.c_char_p),
+    "gs_get_device_count": ([ctypes.POINTER(ctypes.c_int)], ctypes.c_int),
+    "gs_create": ([ctypes.POINTER(ctypes.c_void_p), ctypes.c_int], ctypes.c_int),
+    "gs_destroy": ([ctypes.c_void_p], ctypes.c_int),
+    "gs_set_stream": ([ctypes.c_void_p, ctypes.c_void_p], ctypes.c_int),
+    "gs_synchronize": ([ctypes.c_void_p], ctypes.c_int),
+    "gs_arena_bytes": ([ctypes.c_void_p, ctypes.POINTER(ctypes.c_uint64)], ctypes.c_int),
+    "gs_release_arena": ([ctypes.c_void_p], ctypes.c_int),
+    "gs_sort_keys": ([ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p, ctypes.c_uint64, ctypes.c_int], ctypes.c_int),
+    "gs_sort_pairs": (
+        [ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p, ctypes.c_uint64, ctypes.c_int],
+        ctypes.c_int,
+    ),
+    "gs_sort_keys_host": (
+        [ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p, ctypes.c_uint64, ctypes.c_int],
+        ctypes.c_int,
+    ),
+    "gs_sort_pairs_host": (
+        [ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p, ctypes.c_uint64, ctypes.c_int,
+         ctypes.c_void_p, ctypes.c_uint32, ctypes.c_uint32],
+        ctypes.c_int,
+    ),
+    "gs_sort_segments": (
+        [ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p,
+         ctypes.c_uint32, ctypes.c_int],
+        ctypes.c_int,
+    ),
+    "gs_merge": (
+        [ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_uint64, ctypes.c_void_p,
+         ctypes.c_void_p, ctypes.c_uint64, ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int],
+        ctypes.c_int,
+    ),
+    "gs_bucket_bounds": (
+        [ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_uint32, ctypes.c_void_p,
+         ctypes.c_uint32, ctypes.c_int, ctypes.c_int, ctypes.c_void_p],
+        ctypes.c_int,
+    ),
+    "gs_vrank_sort": (
+        [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p,
+         ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_int, ctypes.POINTER(VrankReport)],
+        ctypes.c_int,
+    ),
+    "gs_gather": (
+        [ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_uint64],
+        ctypes.c_int,
+    ),
+}
+
+
+def exported_symbols() -> list[str]:
+    return sorted(_SIGS)
+
+
+def load_library(path: str | os.PathLike | None = None) -> ctypes.CDLL:
+    """Loads ``libgpusort_hip.so`` (raises if it has not been built)."""
+    global _lib
+    if _lib is not None and path is None:
+        return _lib
+    p = Path(path) if path else LIB_PATH
+    if not p.exists():
+        raise GpusortError(5, f"{p} is missing: build it with `python -m gpusort_amd.build`")
+    try:  # share torch's HIP runtime when torch is present (one runtime per process)
+        import torch  # noqa: F401
+    except Exception:
+        pass
+    lib = ctypes.CDLL(str(p))
+    for name, (args, res) in _SIGS.items():
+        fn = getattr(lib, name)
+        fn.argtypes = args
+        fn.restype = res
+    if path is None:
+        _lib = lib
+    return lib
+
+
+def _check(status: int) -> None:
+    if status != 0:
+        msg = _lib.gs_last_error().decode(errors="replace") if _lib else ""
+        raise GpusortError(status, msg)
+
+
+def device_count() -> int:
+    lib = load_library()
+    n = ctypes.c_int(0)
+    _check(lib.gs_get_device_count(ctypes.byref(n)))
+    return n.value
+
+
+def torch_dtype_code(t) -> int:
+    import torch
+
+    m = {
+        torch.int32: I32,
+        getattr(torch, "uint32", None): U32,
+        torch.float32: F32,
+        torch.int64: I64,
+        getattr(torch, "uint64", None): U64,
+        torch.float64: F64,
+    }
+    code = m.get(t.dtype)
+    if code is None:
+        raise GpusortError(1, f"unsupported tensor dtype {t.dtype}")
+    return code
+
+
+def np_dtype_code(a: np.ndarray) -> int:
+    code = _NP_DTYPE.get(a.dtype)
+    if code is None:
+        raise GpusortError(1, f"unsupported array dtype {a.dtype}")
+    return code
+
+
+def _dev_ptr(t) -> int:
+    if not t.is_cuda:
+        raise GpusortError(1, "expected a device tensor")
+    if not t.is_contiguous():
+        raise GpusortError(1, "expected a contiguous tensor")
+    return t.data_ptr()
+
+
+class Handle:
+    """One library handle: a device, a stream and the HBM arena."""
+
+    def __init__(self, device: int = 0, stream=None):
+        self.lib = load_library()
+        h = ctypes.c_void_p()
+        _check(self.lib.gs_create(ctypes.byref(h), device))
+        self.h = h
+        self.device = device
+        if stream is not None:
+            self.set_stream(stream)
+
+    def close(self):
+        if self.h:
+            self.lib.gs_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_stream(self, stream) -> None:
+        ptr = stream if isinstance(stream, int) else getattr(stream, "cuda_stream", 0)
+        _check(self.lib.gs_set_stream(self.h, ctypes.c_void_p(ptr)))
+
+    def use_torch_stream(self) -> None:
+        import torch
+
+        self.set_stream(torch.cuda.current_stream(self.device).cuda_stream)
+
+    def synchronize(self) -> None:
+        _check(self.lib.gs_synchronize(self.h))
+
+    def arena_bytes(self) -> int:
+        b = ctypes.c_uint64(0)
+        _check(self.lib.gs_arena_bytes(self.h, ctypes.byref(b)))
+        return b.value
+
+    def release_arena(self) -> None:
+        _check(self.lib.gs_release_arena(self.h))
+
+    # ---------------------------------------------------------------- device tensors
+    def sort_keys(self, keys, descending: bool = False) -> None:
+        _check(self.lib.gs_sort_keys(self.h, torch_dtype_code(keys), _dev_ptr(keys), keys.numel(), int(descending)))
+
+    def sort_pairs(self, keys, vals, descending: bool = False) -> None:
+        if keys.numel() != vals.numel():
+            raise GpusortError(1, "keys and values differ in length")
+        _check(
+            self.lib.gs_sort_pairs(
+                self.h, torch_dtype_code(keys), _dev_ptr(keys), torch_dtype_code(vals), _dev_ptr(vals), keys.numel(),
+                int(descending),
+            )
+        )
+
+    def sort_segments(self, keys, offsets, vals=None, descending: bool = False) -> None:
+        nseg = offsets.numel() - 1
+        _check(
+            self.lib.gs_sort_segments(
+                self.h, torch_dtype_code(keys), _dev_ptr(keys), NONE if vals is None else torch_dtype_code(vals),
+                None if vals is None else _dev_ptr(vals), _dev_ptr(offsets), nseg, int(descending),
+            )
+        )
+
+    def merge(self, a_keys, b_keys, out_keys, a_vals=None, b_vals=None, out_vals=None, descending=False) -> None:
+        vcode = NONE if a_vals is None else torch_dtype_code(a_vals)
+        _check(
+            self.lib.gs_merge(
+                self.h, torch_dtype_code(a_keys), _dev_ptr(a_keys), None if a_vals is None else _dev_ptr(a_vals),
+                a_keys.numel(), _dev_ptr(b_keys), None if b_vals is None else _dev_ptr(b_vals), b_keys.numel(),
+                vcode, _dev_ptr(out_keys), None if out_vals is None else _dev_ptr(out_vals), int(descending),
+            )
+        )
+
+    def bucket_bounds(self, keys, run_off: np.ndarray, splitters: np.ndarray, upper=False, descending=False):
+        run_off = np.ascontiguousarray(run_off, dtype=np.uint64)
+        nruns = run_off.size - 1
+        spl = np.ascontiguousarray(splitters)
+        out = np.zeros(nruns * (spl.size + 2), dtype=np.uint64)
+        _check(
+            self.lib.gs_bucket_bounds(
+                self.h, torch_dtype_code(keys), _dev_ptr(keys), run_off.ctypes.data, nruns,
+                spl.ctypes.data if spl.size else None, spl.size, int(upper), int(descending), out.ctypes.data,
+            )
+        )
+        return out.reshape(nruns, spl.size + 2)
+
+    def vrank_sort(self, keys, rank_off: np.ndarray, vals=None, algorithm=HYKSORT, k_way=8, samples_per_rank=32,
+                   descending=False, max_splitters=1 << 16, max_buckets=1 << 20):
+        rank_off = np.ascontiguousarray(rank_off, dtype=np.uint64)
+        p = rank_off.size - 1
+        kcode = torch_dtype_code(keys)
+        spl = np.zeros(max_splitters, dtype=_CODE_NP[kcode])
+        bsz = np.zeros(max_buckets, dtype=np.uint64)
+        fin = np.zeros(p, dtype=np.uint64)
+        rep = VrankReport()
+        rep.max_splitters = max_splitters
+        rep.splitters = spl.ctypes.data
+        rep.bucket_sizes = bsz.ctypes.data_as(ctypes.POINTER(ctypes.c_uint64))
+        rep.max_bucket_sizes = max_buckets
+        rep.final_rank_sizes = fin.ctypes.data_as(ctypes.POINTER(ctypes.c_uint64))
+        _check(
+            self.lib.gs_vrank_sort(
+                self.h, algorithm, kcode, _dev_ptr(keys), NONE if vals is None else torch_dtype_code(vals),
+                None if vals is None else _dev_ptr(vals), rank_off.ctypes.data, p, k_way, samples_per_rank,
+                int(descending), ctypes.byref(rep),
+            )
+        )
+        return {
+            "rounds": rep.rounds,
+            "splitters": spl[: rep.n_splitters].copy(),
+            "bucket_sizes": bsz[: rep.n_bucket_sizes].copy(),
+            "final_rank_sizes": fin.copy(),
+            "max_rank_error": rep.max_rank_error,
+        }
+
+    def gather(self, src, idx, dst) -> None:
+        _check(self.lib.gs_gather(self.h, src.element_size(), _dev_ptr(src), _dev_ptr(idx), _dev_ptr(dst), dst.numel()))
+
+    # ---------------------------------------------------------------- host arrays
+    def sort_keys_host(self, keys: np.ndarray, descending: bool = False) -> None:
+        if not keys.flags.c_contiguous:
+            raise GpusortError(1, "expected a contiguous array")
+        _check(self.lib.gs_sort_keys_host(self.h, np_dtype_code(keys), keys.ctypes.data, keys.size, int(descending)))
+
+    def sort_pairs_host(self, keys: np.ndarray, vals: np.ndarray, descending: bool = False,
+                        pairs_out: np.ndarray | None = None, pair_stride: int = 0, val_offset: int = 0) -> None:
+        _check(
+            self.lib.gs_sort_pairs_host(
+                self.h, np_dtype_code(keys), keys.ctypes.data, np_dtype_code(vals), vals.ctypes.data, keys.size,
+                int(descending), None if pairs_out is None else pairs_out.ctypes.data, pair_stride, val_offset,
+            )
+        )
+
+
+def version() -> str:
+    return load_library().gs_version().decode()

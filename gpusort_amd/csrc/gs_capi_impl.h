@@ -1,0 +1,62 @@
+// Shared host state behind the C ABI: the handle, its HBM arena, the error
+// channel and the device scope guard.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stddef.h>
+#include <stdint.h>
+
+#include "../../include/gpusort_hip.h"
+
+namespace gs {
+
+// Grow-only device buffers, one per use.  A slot is reallocated only when a
+// call needs more bytes than it holds (after the stream drains); steady-state
+// calls of a given size therefore allocate nothing.
+class Arena {
+ public:
+  enum Slot { kRadix = 0, kStageKeys, kStageVals, kStagePairs, kMerge, kSmall, kPipeKeys, kPipeVals, kPipeAux,
+              kSegAux, kSlots };
+  Arena() {
+    for (int i = 0; i < kSlots; ++i) {
+      ptr_[i] = nullptr;
+      cap_[i] = 0;
+    }
+  }
+  ~Arena();
+  hipError_t get(int slot, size_t bytes, hipStream_t stream, void** out);
+  size_t bytes() const;
+  void release();
+
+ private:
+  void* ptr_[kSlots];
+  size_t cap_[kSlots];
+};
+
+class DeviceScope {
+ public:
+  explicit DeviceScope(int dev);
+  ~DeviceScope();
+
+ private:
+  int prev_ = -1;
+  int dev_ = 0;
+};
+
+gs_status fail(gs_status st, const char* fmt, ...);
+gs_status from_hip(hipError_t e, const char* what);
+bool dtype_ok(gs_dtype d);
+int dtype_bytes(gs_dtype d);
+int val_bytes_of(gs_dtype d);
+gs_status sort_device(gs_handle* h, gs_dtype kdt, void* d_keys, gs_dtype vdt, void* d_vals, uint64_t n,
+                      int descending);
+
+}  // namespace gs
+
+struct gs_handle {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  gs::Arena arena;
+  void* pinned = nullptr;
+  size_t pinned_bytes = 0;
+};

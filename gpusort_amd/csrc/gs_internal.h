@@ -1,0 +1,114 @@
+// Host-side internal interface between the C ABI (capi.hip) and the kernel
+// translation units.  Nothing here crosses the library boundary.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stddef.h>
+#include <stdint.h>
+
+#include <vector>
+
+namespace gs {
+
+// Key kinds of the reference's instantiation matrix (src/cuda_utils.cu:257-262).
+enum KeyKind : int { kI32 = 0, kU32 = 1, kF32 = 2, kI64 = 3, kU64 = 4, kF64 = 5 };
+
+inline int key_bytes(int kind) { return kind <= kF32 ? 4 : 8; }
+
+// Transform parameters for an unsigned image of width 4 or 8 bytes; the
+// kernels take the 32- or 64-bit half that matches the key width.
+struct XformParams {
+  uint64_t float_mask;
+  uint64_t sign_flip;
+  uint64_t desc_mask;
+};
+
+XformParams make_xform(int kind, bool descending);
+
+constexpr int kMaxPasses = 8;
+constexpr uint32_t kSkipPass = 0xFFFFFFFFu;
+
+// Device-resident control block of one radix sort: which ping-pong buffer
+// each pass reads (decided on the device from the digit histograms, so no
+// host round trip is needed), the tile tickets of each pass, and the buffer
+// that holds the result.
+struct SortCtrl {
+  uint32_t src[kMaxPasses];
+  uint32_t tile_counter[kMaxPasses];
+  uint32_t final_sel;
+  uint32_t passes_run;
+  uint32_t pad[14];
+};
+
+// ---------------------------------------------------------------- radix sort
+// Onesweep LSD radix sort of n keys (key_bytes 4/8) with an optional payload
+// (val_bytes 0/4/8), stable.  keys/vals are sorted in place; the workspace
+// holds the ping-pong buffers, histograms and look-back status words.
+size_t radix_workspace_bytes(int key_bytes, int val_bytes, uint64_t n);
+hipError_t radix_sort(int key_bytes, int val_bytes, void* keys, void* vals, uint64_t n, const XformParams& xf,
+                      void* workspace, size_t workspace_bytes, hipStream_t stream);
+
+// Tile geometry of the onesweep pass for a key/value width pair.
+uint32_t radix_tile_keys(int key_bytes, int val_bytes);
+
+// ---------------------------------------------------------------- block sort
+// Single-workgroup stable sort of n <= block_sort_max(key_bytes, val_bytes)
+// keys held in LDS (bitonic network over (key, original index) pairs).
+uint32_t block_sort_max(int key_bytes, int val_bytes);
+hipError_t block_sort(int key_bytes, int val_bytes, void* keys, void* vals, uint32_t n, const XformParams& xf,
+                      hipStream_t stream);
+
+// Batched variant: workgroup b sorts segment s = d_seg_ids[b] (or b when
+// d_seg_ids is null), i.e. keys[off[s] .. off[s+1]), which must be no longer
+// than block_sort_max.
+hipError_t block_sort_segments(int key_bytes, int val_bytes, void* keys, void* vals, const uint64_t* d_offsets,
+                               const uint32_t* d_seg_ids, uint32_t nblocks, const XformParams& xf,
+                               hipStream_t stream);
+
+// ---------------------------------------------------------------- merge path
+// One 2-way merge job: out[out_off ..) = merge(a[a_off, a_off+a_len), b[b_off, b_off+b_len)),
+// ties taken from a first (std::merge semantics).  Offsets index the source
+// and destination buffers given to merge_jobs.
+struct MergeJob {
+  uint64_t a_off, a_len, b_off, b_len, out_off;
+};
+
+// Device record of a job: the host record plus the index of its first tile.
+struct MergeJobDev {
+  uint64_t a_off, a_len, b_off, b_len, out_off;
+  uint32_t tile0;
+  uint32_t pad;
+};
+
+// Output elements per merge tile.
+uint32_t merge_tile_elems();
+// Fills out_jobs[0..njobs) and the tile -> job table; returns the tile count.
+uint32_t merge_plan(const MergeJob* jobs, uint32_t njobs, MergeJobDev* out_jobs, std::vector<uint32_t>* tile_job);
+// Runs every job: run a of a job is read from keys_a/vals_a + a_off, run b
+// from keys_b/vals_b + b_off.  d_split needs ntiles entries of scratch.
+hipError_t merge_tiles(int key_bytes, int val_bytes, const void* keys_a, const void* keys_b, const void* vals_a,
+                       const void* vals_b, void* dst_keys, void* dst_vals, const MergeJobDev* d_jobs,
+                       const uint32_t* d_tile_job, uint32_t ntiles, const XformParams& xf, uint64_t* d_split,
+                       hipStream_t stream);
+
+// ---------------------------------------------------------------- splitting
+// Bucket boundaries of sorted runs.  Run r is keys[run_off[r] .. run_off[r+1])
+// and is cut by splitters[spl_first[r] .. spl_first[r] + nspl).  Output:
+// bounds[r*(nspl+2) + j], j = 0..nspl+1: run start, then run start +
+// lower_bound (upper_bound if `upper`) of each splitter, then run end.
+hipError_t bucket_bounds(int key_bytes, const void* keys, const uint64_t* d_run_off, uint32_t nruns,
+                         const void* d_splitters, const uint32_t* d_spl_first, uint32_t nspl, int upper,
+                         const XformParams& xf, uint64_t* d_bounds, hipStream_t stream);
+
+// ---------------------------------------------------------------- utilities
+hipError_t fill_u32(uint32_t* p, uint64_t n, uint32_t v, hipStream_t stream);
+hipError_t copy_bytes(void* dst, const void* src, uint64_t bytes, hipStream_t stream);
+hipError_t gather_values(int val_bytes, const void* src, const uint64_t* idx, void* dst, uint64_t n,
+                         hipStream_t stream);
+hipError_t pack_pairs(int key_bytes, const void* keys, int val_bytes, const void* vals, void* out, uint64_t n,
+                      uint32_t stride_bytes, uint32_t val_offset, hipStream_t stream);
+hipError_t iota_u64(uint64_t* out, uint64_t n, uint64_t base, hipStream_t stream);
+
+inline size_t align_up(size_t x, size_t a) { return (x + a - 1) / a * a; }
+
+}  // namespace gs

@@ -1,0 +1,531 @@
+// Onesweep LSD radix sort for gfx950 (wave64, no MFMA).
+//
+// Replaces the device sort of the reference's local-sort seam:
+//   thrust::sort         (src/cuda_utils.cu:424)       -- keys only
+//   thrust::sort_by_key  (src/cuda_utils.cu:477-478)   -- stable, with payload
+//
+// One sort is 3 + P launches on one stream, with no host synchronisation:
+//   fill      zero the global digit histograms
+//   hist      one read of all keys -> the 256-bin histogram of every digit
+//             position, and zero of every pass's look-back status words
+//   scan      exclusive digit offsets per pass; passes whose digit is the same
+//             for every key are marked skipped and the ping-pong buffer of
+//             every remaining pass is chosen on the device
+//   pass x P  onesweep pass (below)
+//   copy      only if the result ended in the temporary buffer
+//
+// Onesweep pass, per tile of THREADS*ITEMS keys (tile order from an atomic
+// ticket so every tile's predecessors are already resident):
+//   1. coalesced wave-striped loads (lane-consecutive addresses per item)
+//   2. stable ranking inside each wave: ballot match of the 8-bit digit, a
+//      per-wave LDS digit counter updated by the lowest matching lane
+//   3. per-digit scan across waves and digits -> local slot of every key;
+//      the tile's digit counts are published at once (decoupled look-back)
+//   4. keys are written to LDS at their local slot (tile sorted by digit)
+//   5. 256 look-back lanes accumulate predecessor counts until an inclusive
+//      prefix is found, then publish this tile's inclusive prefix
+//   6. LDS is read back in slot order and written to global memory: lanes
+//      that hold the same digit write consecutive addresses
+//   7. payloads follow the same slots through LDS.
+#include "gs_device.h"
+#include "gs_internal.h"
+
+namespace gs {
+
+constexpr int kRadix = 256;
+
+template <typename U>
+__device__ inline void hist_add(uint32_t* h, uint32_t d) {
+  // Many lanes of a skewed input hit the same LDS word; when the whole wave
+  // carries one digit a single add replaces 64 serialised atomics.
+  const uint64_t active = __builtin_amdgcn_read_exec();
+  const uint32_t d0 = __builtin_amdgcn_readfirstlane(d);
+  if (ballot(d == d0) == active) {
+    if (count_below(active) == 0) atomicAdd(&h[d0], (uint32_t)__popcll(active));
+  } else {
+    atomicAdd(&h[d], 1u);
+  }
+}
+
+__global__ void fill_u32_kernel(uint32_t* p, uint64_t n, uint32_t v) {
+  const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) p[i] = v;
+}
+
+template <typename U>
+struct Vec16;
+template <>
+struct Vec16<uint32_t> {
+  using T = uint4;
+  static constexpr int N = 4;
+  __device__ static uint32_t get(const T& v, int i) { return i == 0 ? v.x : i == 1 ? v.y : i == 2 ? v.z : v.w; }
+};
+template <>
+struct Vec16<uint64_t> {
+  using T = ulonglong2;
+  static constexpr int N = 2;
+  __device__ static uint64_t get(const T& v, int i) { return i == 0 ? v.x : v.y; }
+};
+
+template <typename U>
+__global__ __launch_bounds__(256) void radix_hist_kernel(const U* __restrict__ keys, uint64_t n, Xform<U> xf,
+                                                         int passes, unsigned long long* __restrict__ g_hist,
+                                                         uint4* __restrict__ zero_ptr, uint64_t zero_vec4) {
+  __shared__ uint32_t s_hist[kMaxPasses * kRadix];
+  const uint32_t tid = threadIdx.x;
+  for (uint32_t i = tid; i < (uint32_t)passes * kRadix; i += 256) s_hist[i] = 0;
+  const uint64_t gtid = (uint64_t)blockIdx.x * 256 + tid;
+  const uint64_t gstride = (uint64_t)gridDim.x * 256;
+  for (uint64_t i = gtid; i < zero_vec4; i += gstride) zero_ptr[i] = make_uint4(0, 0, 0, 0);
+  __syncthreads();
+
+  using VT = typename Vec16<U>::T;
+  constexpr int VN = Vec16<U>::N;
+  const uint64_t nvec = n / VN;
+  const VT* kv = reinterpret_cast<const VT*>(keys);
+  for (uint64_t vi = gtid; vi < nvec; vi += gstride) {
+    const VT v = kv[vi];
+#pragma unroll
+    for (int e = 0; e < VN; ++e) {
+      const U u = to_ordered(Vec16<U>::get(v, e), xf);
+      for (int p = 0; p < passes; ++p) hist_add<U>(s_hist + p * kRadix, digit_of(u, p * 8));
+    }
+  }
+  if (blockIdx.x == 0) {
+    for (uint64_t i = nvec * VN + tid; i < n; i += 256) {
+      const U u = to_ordered(keys[i], xf);
+      for (int p = 0; p < passes; ++p) atomicAdd(&s_hist[p * kRadix + digit_of(u, p * 8)], 1u);
+    }
+  }
+  __syncthreads();
+  for (uint32_t i = tid; i < (uint32_t)passes * kRadix; i += 256)
+    if (s_hist[i]) atomicAdd(&g_hist[i], (unsigned long long)s_hist[i]);
+}
+
+__global__ __launch_bounds__(256) void radix_scan_kernel(const unsigned long long* __restrict__ g_hist,
+                                                         uint64_t* __restrict__ digit_base, SortCtrl* ctrl,
+                                                         int passes, uint64_t n) {
+  __shared__ uint64_t s_wsum[4];
+  __shared__ uint32_t s_trivial[kMaxPasses];
+  const uint32_t tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  for (int p = 0; p < passes; ++p) {
+    const uint64_t c = g_hist[p * kRadix + tid];
+    const uint64_t incl = wave_incl_scan64(c);
+    if (lane == 63) s_wsum[wave] = incl;
+    __syncthreads();
+    uint64_t add = 0;
+    for (uint32_t w = 0; w < wave; ++w) add += s_wsum[w];
+    digit_base[p * kRadix + tid] = incl - c + add;
+    const int trivial = __syncthreads_or(c == n);
+    if (tid == 0) s_trivial[p] = trivial;
+    __syncthreads();
+  }
+  if (tid == 0) {
+    uint32_t sel = 0, run = 0;
+    for (int p = 0; p < kMaxPasses; ++p) {
+      ctrl->tile_counter[p] = 0;
+      if (p >= passes || s_trivial[p]) {
+        ctrl->src[p] = kSkipPass;
+      } else {
+        ctrl->src[p] = sel;
+        sel ^= 1u;
+        ++run;
+      }
+    }
+    ctrl->final_sel = sel;
+    ctrl->passes_run = run;
+  }
+}
+
+template <typename U, typename V, typename S>
+struct OnesweepArgs {
+  U* keys0;
+  U* keys1;
+  V* vals0;
+  V* vals1;
+  uint64_t n;
+  Xform<U> xf;
+  int pass;
+  int shift;
+  const uint64_t* digit_base;  // 256 offsets of this pass
+  S* status;                   // tiles*256 status words of this pass, zeroed
+  SortCtrl* ctrl;
+};
+
+template <typename U, typename V, bool KV, int THREADS, int ITEMS, typename S>
+__global__ __launch_bounds__(THREADS) void onesweep_kernel(OnesweepArgs<U, V, S> a) {
+  constexpr int WAVES = THREADS / kWave;
+  constexpr uint32_t TILE = THREADS * ITEMS;
+  constexpr int FLAG_SHIFT = sizeof(S) * 8 - 2;
+  constexpr S FLAG_AGG = S(1) << FLAG_SHIFT;
+  constexpr S FLAG_PRE = S(2) << FLAG_SHIFT;
+  constexpr S VAL_MASK = FLAG_AGG - 1;
+  constexpr size_t BUF_ELEM = (sizeof(U) > sizeof(V) || !KV) ? sizeof(U) : sizeof(V);
+
+  __shared__ uint32_t s_wave_hist[WAVES * kRadix];
+  __shared__ uint32_t s_local_start[kRadix];
+  __shared__ uint64_t s_gofs[kRadix];
+  __shared__ uint32_t s_wsum[4];
+  __shared__ uint32_t s_tile;
+  __shared__ __attribute__((aligned(16))) unsigned char s_buf[TILE * BUF_ELEM];
+
+  const uint32_t src = a.ctrl->src[a.pass];
+  if (src > 1u) return;  // every key has the same digit here: the pass is the identity
+  const U* __restrict__ kin = src ? a.keys1 : a.keys0;
+  U* __restrict__ kout = src ? a.keys0 : a.keys1;
+  const V* __restrict__ vin = src ? a.vals1 : a.vals0;
+  V* __restrict__ vout = src ? a.vals0 : a.vals1;
+
+  const uint32_t tid = threadIdx.x;
+  const uint32_t lane = tid & 63u;
+  const uint32_t wave = tid >> 6;
+
+  if (tid == 0) s_tile = atomicAdd(&a.ctrl->tile_counter[a.pass], 1u);
+  for (uint32_t i = tid; i < WAVES * kRadix; i += THREADS) s_wave_hist[i] = 0;
+  __syncthreads();
+  const uint32_t tile = s_tile;
+  const uint64_t tile_base = (uint64_t)tile * TILE;
+  const uint64_t remain = a.n - tile_base;
+  const uint32_t valid = remain < TILE ? (uint32_t)remain : TILE;
+
+  // 1. wave-striped loads: item i of lane l of wave w is key
+  //    tile_base + w*64*ITEMS + i*64 + l, so item order within a wave is key order.
+  U k[ITEMS];
+  V v[KV ? ITEMS : 1];
+  uint32_t r[ITEMS];
+  const uint64_t wbase = tile_base + (uint64_t)wave * (kWave * ITEMS) + lane;
+  if (valid == TILE) {
+#pragma unroll
+    for (int i = 0; i < ITEMS; ++i) k[i] = kin[wbase + (uint64_t)i * kWave];
+    if constexpr (KV) {
+#pragma unroll
+      for (int i = 0; i < ITEMS; ++i) v[i] = vin[wbase + (uint64_t)i * kWave];
+    }
+  } else {
+#pragma unroll
+    for (int i = 0; i < ITEMS; ++i) {
+      const uint64_t idx = wbase + (uint64_t)i * kWave;
+      k[i] = idx < a.n ? kin[idx] : U(0);
+    }
+    if constexpr (KV) {
+#pragma unroll
+      for (int i = 0; i < ITEMS; ++i) {
+        const uint64_t idx = wbase + (uint64_t)i * kWave;
+        v[i] = idx < a.n ? vin[idx] : V(0);
+      }
+    }
+  }
+
+  // Keys past the end of the array take digit 255: they are the last keys of
+  // the tile in key order, so they land on the last (TILE - valid) slots.
+  const bool full = valid == TILE;
+  auto digit_at = [&](int i) -> uint32_t {
+    const uint32_t d = digit_of(to_ordered(k[i], a.xf), a.shift);
+    if (full) return d;
+    return (wbase + (uint64_t)i * kWave) < a.n ? d : 255u;
+  };
+
+  // 2. stable rank inside the wave.
+  uint32_t* wh = s_wave_hist + wave * kRadix;
+#pragma unroll
+  for (int i = 0; i < ITEMS; ++i) {
+    const uint32_t d = digit_at(i);
+    const uint64_t peers = match_any<8>(d);
+    const uint32_t below = count_below(peers);
+    const uint32_t old = wh[d];
+    __builtin_amdgcn_wave_barrier();
+    if (below == 0) wh[d] = old + (uint32_t)__popcll(peers);
+    __builtin_amdgcn_wave_barrier();
+    r[i] = old + below;
+  }
+  __syncthreads();
+
+  // 3. cross-wave offsets per digit, tile digit totals, publish aggregate.
+  uint32_t total = 0, pub = 0;
+  if (tid < kRadix) {
+#pragma unroll
+    for (int w = 0; w < WAVES; ++w) {
+      const uint32_t c = s_wave_hist[w * kRadix + tid];
+      s_wave_hist[w * kRadix + tid] = total;
+      total += c;
+    }
+    pub = total - ((tid == kRadix - 1) ? (TILE - valid) : 0u);
+    const S word = (tile == 0) ? (FLAG_PRE | (S)(a.digit_base[tid] + pub)) : (FLAG_AGG | (S)pub);
+    status_store(&a.status[(uint64_t)tile * kRadix + tid], word);
+  }
+  uint32_t incl = 0;
+  if (tid < kRadix) {
+    incl = wave_incl_scan(total);
+    if (lane == 63) s_wsum[wave] = incl;
+  }
+  __syncthreads();
+  if (tid < kRadix) {
+    uint32_t add = 0;
+    for (uint32_t w = 0; w < wave; ++w) add += s_wsum[w];
+    s_local_start[tid] = incl - total + add;
+  }
+  __syncthreads();
+
+  // 4. local slots; tile sorted by digit in LDS.
+  U* s_k = reinterpret_cast<U*>(s_buf);
+#pragma unroll
+  for (int i = 0; i < ITEMS; ++i) {
+    const uint32_t d = digit_at(i);
+    r[i] += s_local_start[d] + s_wave_hist[wave * kRadix + d];
+    s_k[r[i]] = k[i];
+  }
+
+  // 5. decoupled look-back, one lane per digit.
+  if (tid < kRadix) {
+    uint64_t excl;
+    if (tile == 0) {
+      excl = a.digit_base[tid];
+    } else {
+      excl = 0;
+      int64_t t = (int64_t)tile - 1;
+      for (;;) {
+        const S w = status_load(&a.status[(uint64_t)t * kRadix + tid]);
+        const S flag = w & ~VAL_MASK;
+        if (flag == 0) {
+          __builtin_amdgcn_s_sleep(1);
+          continue;
+        }
+        excl += (uint64_t)(w & VAL_MASK);
+        if (flag == FLAG_PRE) break;
+        --t;
+      }
+      status_store(&a.status[(uint64_t)tile * kRadix + tid], (S)(FLAG_PRE | (S)(excl + pub)));
+    }
+    s_gofs[tid] = excl - (uint64_t)s_local_start[tid];
+  }
+  __syncthreads();
+
+  // 6. coalesced scatter of keys in slot order.
+  uint32_t dig[KV ? ITEMS : 1];
+#pragma unroll
+  for (int j = 0; j < ITEMS; ++j) {
+    const uint32_t p = (uint32_t)j * THREADS + tid;
+    if (p < valid) {
+      const U key = s_k[p];
+      const uint32_t d = digit_of(to_ordered(key, a.xf), a.shift);
+      kout[s_gofs[d] + p] = key;
+      if constexpr (KV) dig[j] = d;
+    }
+  }
+
+  // 7. payloads through the same slots.
+  if constexpr (KV) {
+    __syncthreads();
+    V* s_v = reinterpret_cast<V*>(s_buf);
+#pragma unroll
+    for (int i = 0; i < ITEMS; ++i) s_v[r[i]] = v[i];
+    __syncthreads();
+#pragma unroll
+    for (int j = 0; j < ITEMS; ++j) {
+      const uint32_t p = (uint32_t)j * THREADS + tid;
+      if (p < valid) vout[s_gofs[dig[j]] + p] = s_v[p];
+    }
+  }
+}
+
+template <typename U, typename V, bool KV>
+__global__ __launch_bounds__(256) void radix_final_copy_kernel(U* __restrict__ k0, const U* __restrict__ k1,
+                                                               V* __restrict__ v0, const V* __restrict__ v1,
+                                                               uint64_t n, const SortCtrl* ctrl) {
+  if (ctrl->final_sel == 0) return;
+  const uint64_t stride = (uint64_t)gridDim.x * 256;
+  for (uint64_t i = (uint64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += stride) {
+    k0[i] = k1[i];
+    if constexpr (KV) v0[i] = v1[i];
+  }
+}
+
+// ------------------------------------------------------------------ host side
+
+template <typename U, int VB>
+struct Geometry;
+// Tile sizes: 512 threads (8 waves); LDS holds one tile of the wider of key
+// and payload, reused for the payload pass.
+template <>
+struct Geometry<uint32_t, 0> {
+  static constexpr int THREADS = 512, ITEMS = 16;
+};
+template <>
+struct Geometry<uint32_t, 4> {
+  static constexpr int THREADS = 512, ITEMS = 16;
+};
+template <>
+struct Geometry<uint32_t, 8> {
+  static constexpr int THREADS = 512, ITEMS = 12;
+};
+template <>
+struct Geometry<uint64_t, 0> {
+  static constexpr int THREADS = 512, ITEMS = 12;
+};
+template <>
+struct Geometry<uint64_t, 4> {
+  static constexpr int THREADS = 512, ITEMS = 12;
+};
+template <>
+struct Geometry<uint64_t, 8> {
+  static constexpr int THREADS = 512, ITEMS = 12;
+};
+
+template <int VB>
+struct ValT {
+  using T = uint32_t;
+};
+template <>
+struct ValT<8> {
+  using T = uint64_t;
+};
+
+static inline uint32_t tile_keys_for(int kb, int vb) {
+  if (kb == 4) {
+    if (vb == 0) return Geometry<uint32_t, 0>::THREADS * Geometry<uint32_t, 0>::ITEMS;
+    if (vb == 4) return Geometry<uint32_t, 4>::THREADS * Geometry<uint32_t, 4>::ITEMS;
+    return Geometry<uint32_t, 8>::THREADS * Geometry<uint32_t, 8>::ITEMS;
+  }
+  if (vb == 0) return Geometry<uint64_t, 0>::THREADS * Geometry<uint64_t, 0>::ITEMS;
+  if (vb == 4) return Geometry<uint64_t, 4>::THREADS * Geometry<uint64_t, 4>::ITEMS;
+  return Geometry<uint64_t, 8>::THREADS * Geometry<uint64_t, 8>::ITEMS;
+}
+
+uint32_t radix_tile_keys(int key_bytes, int val_bytes) { return tile_keys_for(key_bytes, val_bytes); }
+
+struct RadixLayout {
+  size_t off_keys_tmp, off_vals_tmp, off_ctrl, off_hist, off_base, off_status, total;
+  uint64_t tiles;
+  int passes;
+  size_t status_word;
+};
+
+static RadixLayout radix_layout(int kb, int vb, uint64_t n) {
+  RadixLayout L{};
+  const uint32_t tile = tile_keys_for(kb, vb);
+  L.tiles = (n + tile - 1) / tile;
+  L.passes = kb;  // 8-bit digits: one pass per key byte
+  L.status_word = (n < (1ull << 30)) ? 4 : 8;
+  size_t o = 0;
+  L.off_keys_tmp = o;
+  o = align_up(o + n * kb, 256);
+  L.off_vals_tmp = o;
+  o = align_up(o + n * vb, 256);
+  L.off_ctrl = o;
+  o = align_up(o + sizeof(SortCtrl), 256);
+  L.off_hist = o;
+  o = align_up(o + kMaxPasses * kRadix * sizeof(unsigned long long), 256);
+  L.off_base = o;
+  o = align_up(o + kMaxPasses * kRadix * sizeof(uint64_t), 256);
+  L.off_status = o;
+  o = align_up(o + (size_t)L.passes * L.tiles * kRadix * L.status_word, 256);
+  L.total = o;
+  return L;
+}
+
+size_t radix_workspace_bytes(int key_bytes, int val_bytes, uint64_t n) {
+  return radix_layout(key_bytes, val_bytes, n).total;
+}
+
+template <typename U>
+static Xform<U> xform_of(const XformParams& p) {
+  return Xform<U>{(U)p.float_mask, (U)p.sign_flip, (U)p.desc_mask};
+}
+
+template <typename U, int VB, typename S>
+static hipError_t radix_sort_t(U* keys, void* vals_v, uint64_t n, const XformParams& xp, unsigned char* ws,
+                               const RadixLayout& L, hipStream_t stream) {
+  using V = typename ValT<VB>::T;
+  constexpr bool KV = VB != 0;
+  constexpr int THREADS = Geometry<U, VB>::THREADS;
+  constexpr int ITEMS = Geometry<U, VB>::ITEMS;
+  const Xform<U> xf = xform_of<U>(xp);
+  U* keys_tmp = reinterpret_cast<U*>(ws + L.off_keys_tmp);
+  V* vals = reinterpret_cast<V*>(vals_v);
+  V* vals_tmp = reinterpret_cast<V*>(ws + L.off_vals_tmp);
+  SortCtrl* ctrl = reinterpret_cast<SortCtrl*>(ws + L.off_ctrl);
+  unsigned long long* hist = reinterpret_cast<unsigned long long*>(ws + L.off_hist);
+  uint64_t* base = reinterpret_cast<uint64_t*>(ws + L.off_base);
+  S* status = reinterpret_cast<S*>(ws + L.off_status);
+  const uint64_t status_per_pass = L.tiles * kRadix;
+
+  hipLaunchKernelGGL(fill_u32_kernel, dim3(8), dim3(256), 0, stream, reinterpret_cast<uint32_t*>(hist),
+                     (uint64_t)(kMaxPasses * kRadix * 2), 0u);
+  const uint64_t status_bytes = (uint64_t)L.passes * status_per_pass * sizeof(S);
+  const uint64_t nvec = n / Vec16<U>::N;
+  uint64_t hist_blocks = (nvec + 256 * 8 - 1) / (256 * 8);
+  if (hist_blocks < 1) hist_blocks = 1;
+  if (hist_blocks > 2048) hist_blocks = 2048;
+  hipLaunchKernelGGL(radix_hist_kernel<U>, dim3((uint32_t)hist_blocks), dim3(256), 0, stream, keys, n, xf,
+                     L.passes, hist, reinterpret_cast<uint4*>(status), status_bytes / 16);
+  hipLaunchKernelGGL(radix_scan_kernel, dim3(1), dim3(256), 0, stream, hist, base, ctrl, L.passes, n);
+  for (int p = 0; p < L.passes; ++p) {
+    OnesweepArgs<U, V, S> a;
+    a.keys0 = keys;
+    a.keys1 = keys_tmp;
+    a.vals0 = vals;
+    a.vals1 = vals_tmp;
+    a.n = n;
+    a.xf = xf;
+    a.pass = p;
+    a.shift = 8 * p;
+    a.digit_base = base + p * kRadix;
+    a.status = status + p * status_per_pass;
+    a.ctrl = ctrl;
+    hipLaunchKernelGGL((onesweep_kernel<U, V, KV, THREADS, ITEMS, S>), dim3((uint32_t)L.tiles), dim3(THREADS), 0,
+                       stream, a);
+  }
+  uint64_t copy_blocks = (n + 255) / 256;
+  if (copy_blocks > 4096) copy_blocks = 4096;
+  hipLaunchKernelGGL((radix_final_copy_kernel<U, V, KV>), dim3((uint32_t)copy_blocks), dim3(256), 0, stream, keys,
+                     keys_tmp, vals, vals_tmp, n, ctrl);
+  return hipGetLastError();
+}
+
+template <typename U, int VB>
+static hipError_t radix_sort_s(U* keys, void* vals, uint64_t n, const XformParams& xp, unsigned char* ws,
+                               const RadixLayout& L, hipStream_t stream) {
+  if (L.status_word == 4) return radix_sort_t<U, VB, uint32_t>(keys, vals, n, xp, ws, L, stream);
+  return radix_sort_t<U, VB, uint64_t>(keys, vals, n, xp, ws, L, stream);
+}
+
+hipError_t radix_sort(int kb, int vb, void* keys, void* vals, uint64_t n, const XformParams& xf, void* workspace,
+                      size_t workspace_bytes, hipStream_t stream) {
+  if (n == 0) return hipSuccess;
+  const RadixLayout L = radix_layout(kb, vb, n);
+  if (workspace_bytes < L.total) return hipErrorInvalidValue;
+  unsigned char* ws = reinterpret_cast<unsigned char*>(workspace);
+  if (kb == 4) {
+    uint32_t* k = reinterpret_cast<uint32_t*>(keys);
+    if (vb == 0) return radix_sort_s<uint32_t, 0>(k, vals, n, xf, ws, L, stream);
+    if (vb == 4) return radix_sort_s<uint32_t, 4>(k, vals, n, xf, ws, L, stream);
+    if (vb == 8) return radix_sort_s<uint32_t, 8>(k, vals, n, xf, ws, L, stream);
+  } else if (kb == 8) {
+    uint64_t* k = reinterpret_cast<uint64_t*>(keys);
+    if (vb == 0) return radix_sort_s<uint64_t, 0>(k, vals, n, xf, ws, L, stream);
+    if (vb == 4) return radix_sort_s<uint64_t, 4>(k, vals, n, xf, ws, L, stream);
+    if (vb == 8) return radix_sort_s<uint64_t, 8>(k, vals, n, xf, ws, L, stream);
+  }
+  return hipErrorInvalidValue;
+}
+
+hipError_t fill_u32(uint32_t* p, uint64_t n, uint32_t v, hipStream_t stream) {
+  uint64_t blocks = (n + 255) / 256;
+  if (blocks > 4096) blocks = 4096;
+  if (blocks == 0) return hipSuccess;
+  hipLaunchKernelGGL(fill_u32_kernel, dim3((uint32_t)blocks), dim3(256), 0, stream, p, n, v);
+  return hipGetLastError();
+}
+
+XformParams make_xform(int kind, bool descending) {
+  XformParams x{};
+  const bool wide = key_bytes(kind) == 8;
+  const uint64_t msb = wide ? (1ull << 63) : (1ull << 31);
+  const uint64_t all = wide ? ~0ull : 0xFFFFFFFFull;
+  x.float_mask = (kind == kF32 || kind == kF64) ? all : 0;
+  x.sign_flip = (kind == kU32 || kind == kU64) ? 0 : msb;
+  x.desc_mask = descending ? all : 0;
+  return x;
+}
+
+}  // namespace gs

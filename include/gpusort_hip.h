@@ -1,0 +1,171 @@
+/*
+ * gpusort_hip.h -- C ABI of the MI355X (gfx950) per-rank sort path.
+ *
+ * This is the drop-in seam for the reference's GPU local sort and per-rank
+ * stages.  Plain pointers and sizes only; every entry returns a gs_status and
+ * never throws.  The C++ headers under include/gpusort/ wrap these entries in
+ * the reference's template API (HykSort / SampleSort / HyperQuickSort /
+ * CudaUtils / ParUtils) and re-throw errors as the reference's exceptions.
+ *
+ * Citations are files of monotone-RK/gpusort.
+ */
+#ifndef GPUSORT_HIP_H_
+#define GPUSORT_HIP_H_
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Status codes.  The reference reports errors as ThrustException /
+ * MpiException messages (src/cuda_utils.cu:373-393, :415-441,
+ * src/mpi_impl.cc:219-266); gs_last_error() returns the message. */
+typedef enum gs_status {
+  GS_OK = 0,
+  GS_INVALID_ARG = 1,
+  GS_OUT_OF_MEMORY = 2,
+  GS_ADMISSION_REFUSED = 3,
+  GS_DEVICE_ERROR = 4,
+  GS_NO_DEVICE = 5,
+  GS_UNSUPPORTED = 6,
+  GS_EMPTY_INPUT = 7
+} gs_status;
+
+/* Key / value element types: the primitive types of the reference's
+ * instantiation matrix (src/cuda_utils.cu:257-262, :288-318). */
+typedef enum gs_dtype {
+  GS_I32 = 0,
+  GS_U32 = 1,
+  GS_F32 = 2,
+  GS_I64 = 3,
+  GS_U64 = 4,
+  GS_F64 = 5,
+  GS_NONE = 255 /* no payload */
+} gs_dtype;
+
+/* Per-rank algorithms of the vector API. */
+typedef enum gs_algorithm {
+  GS_HYKSORT = 0,       /* include/helpers/hyksort_helper.h */
+  GS_SAMPLESORT = 1,    /* include/helpers/samplesort_helper.h */
+  GS_HYPERQUICKSORT = 2 /* include/helpers/hyperquicksort_helper.h */
+} gs_algorithm;
+
+typedef struct gs_handle gs_handle;
+
+/* Library version string. */
+const char* gs_version(void);
+
+/* Message of the last failed call on this thread ("" if none). */
+const char* gs_last_error(void);
+
+/* Number of visible HIP devices.
+ * Replaces CudaUtils::GetDeviceCount (include/cuda/cuda_utils.h:12,
+ * src/cuda_utils.cu:81-88); IsHavingDevices is count > 0 (cuda_utils.cu:90-92). */
+gs_status gs_get_device_count(int* count);
+
+/* Creates a handle bound to `device` (its stream is the null stream until
+ * gs_set_stream).  The handle owns the HBM workspace arena reused by every
+ * call; device-pointer entries allocate only when the arena must grow.
+ * Replaces SelectDevice (src/cuda_utils.cu:395-401). */
+gs_status gs_create(gs_handle** out, int device);
+gs_status gs_destroy(gs_handle* h);
+gs_status gs_set_stream(gs_handle* h, void* hip_stream);
+gs_status gs_synchronize(gs_handle* h);
+/* Bytes currently held by the handle's arena. */
+gs_status gs_arena_bytes(gs_handle* h, uint64_t* bytes);
+/* Frees the arena (the next call re-grows it). */
+gs_status gs_release_arena(gs_handle* h);
+
+/* Sorts n keys of type kdt in device memory in place, ascending (descending
+ * if `descending`), stream-ordered on the handle's stream.
+ * Replaces the device sort of SortByThrust (src/cuda_utils.cu:403-445):
+ * thrust::sort(d_v, less|greater) at :424.  Float keys: finite values only;
+ * -0.0 orders before +0.0. */
+gs_status gs_sort_keys(gs_handle* h, gs_dtype kdt, void* d_keys, uint64_t n, int descending);
+
+/* Stable key-value sort of device arrays in place (payload vdt of 4 or 8
+ * bytes follows its key; equal keys keep their input order).
+ * Replaces thrust::sort_by_key (src/cuda_utils.cu:447-506, :477). */
+gs_status gs_sort_pairs(gs_handle* h, gs_dtype kdt, void* d_keys, gs_dtype vdt, void* d_vals, uint64_t n,
+                        int descending);
+
+/* Host-memory forms: copy in, sort, copy out, synchronised before return.
+ * gs_sort_keys_host replaces CudaUtils::Sort(comp, rank, first, last)
+ * (include/cuda/cuda_utils.h:16-19, src/cuda_utils.cu:97-167).
+ * gs_sort_pairs_host replaces CudaUtils::SortByKey(comp, rank, kf, kl, vf,
+ * _out) (include/cuda/cuda_utils.h:21-27, src/cuda_utils.cu:169-243): keys
+ * and values are sorted in place and, if h_pairs_out is not NULL, the pairs
+ * are also written there as records of pair_stride bytes with the key at
+ * offset 0 and the value at val_offset (the std::pair<K,V> layout of
+ * src/cuda_utils.cu:498-502). */
+gs_status gs_sort_keys_host(gs_handle* h, gs_dtype kdt, void* h_keys, uint64_t n, int descending);
+gs_status gs_sort_pairs_host(gs_handle* h, gs_dtype kdt, void* h_keys, gs_dtype vdt, void* h_vals, uint64_t n,
+                             int descending, void* h_pairs_out, uint32_t pair_stride, uint32_t val_offset);
+
+/* Batched segmented sort: segment s is [d_offsets[s], d_offsets[s+1]) of the
+ * device arrays (offsets in device memory, nseg+1 entries, non-decreasing).
+ * Every segment is sorted independently (stable with payload) in one call.
+ * d_vals may be NULL with vdt GS_NONE.  No reference counterpart as one call:
+ * it batches many CudaUtils::Sort / SortByKey calls (src/cuda_utils.cu:97-243). */
+gs_status gs_sort_segments(gs_handle* h, gs_dtype kdt, void* d_keys, gs_dtype vdt, void* d_vals,
+                           const uint64_t* d_offsets, uint32_t nseg, int descending);
+
+/* 2-way merge of sorted device runs a (na) and b (nb) into out (na+nb), ties
+ * from a first.  Replaces OmpUtils::Merge (include/omp/omp_utils.h:54-145). */
+gs_status gs_merge(gs_handle* h, gs_dtype kdt, const void* d_a_keys, const void* d_a_vals, uint64_t na,
+                   const void* d_b_keys, const void* d_b_vals, uint64_t nb, gs_dtype vdt, void* d_out_keys,
+                   void* d_out_vals, int descending);
+
+/* Bucket boundaries of nruns sorted device runs (run r = [run_off[r],
+ * run_off[r+1]) of d_keys; h_run_off on the host, nruns+1 entries) cut by
+ * nspl host splitters shared by all runs.  h_bounds receives nruns*(nspl+2)
+ * offsets: run start, the lower_bound (upper_bound if `upper`) of each
+ * splitter, run end.
+ * Replaces the split loops of HykSortHelper::TransferData
+ * (include/helpers/hyksort_helper.h:563-568, lower_bound) and
+ * SampleSortHelper::TransferData (include/helpers/samplesort_helper.h:525-537,
+ * upper_bound). */
+gs_status gs_bucket_bounds(gs_handle* h, gs_dtype kdt, const void* d_keys, const uint64_t* h_run_off, uint32_t nruns,
+                           const void* h_splitters, uint32_t nspl, int upper, int descending, uint64_t* h_bounds);
+
+/* Report of one virtual-rank run (filled by gs_vrank_sort). */
+typedef struct gs_vrank_report {
+  uint32_t rounds;             /* exchange rounds run */
+  uint32_t max_splitters;      /* capacity of `splitters` (in keys) given by the caller */
+  uint32_t n_splitters;        /* splitters written */
+  void* splitters;             /* caller buffer, key type: every round's splitters, group by group */
+  uint64_t* bucket_sizes;      /* caller buffer: per round, per source rank, per bucket */
+  uint32_t max_bucket_sizes;   /* capacity of bucket_sizes */
+  uint32_t n_bucket_sizes;     /* entries written */
+  uint64_t* final_rank_sizes;  /* caller buffer, p entries: keys held by each virtual rank at the end */
+  double max_rank_error;       /* max |rank(splitter) - target| / N over all rounds (HykSort quality bound) */
+} gs_vrank_report;
+
+/* Per-rank pipeline of HykSort / SampleSort / HyperQuickSort over p virtual
+ * ranks held as device segments of one array.  Rank r starts with keys
+ * [h_rank_off[r], h_rank_off[r+1]) (host offsets, p+1 entries).  Runs the
+ * local sort of every rank, then the algorithm's rounds: splitters from
+ * `samples_per_rank` regular samples, bucket bounds, the virtual exchange
+ * (counts -> offsets -> addressing) and the k-way merge of received runs.
+ * On return d_keys/d_vals hold the global sorted order, rank by rank; a
+ * stable sort when d_vals carries the original index.
+ * Replaces SortAmongProcesses (include/helpers/hyksort_helper.h:453-536,
+ * include/helpers/samplesort_helper.h:420-501,
+ * include/helpers/hyperquicksort_helper.h:261-346) with the MPI exchange
+ * replaced by device addressing. */
+gs_status gs_vrank_sort(gs_handle* h, gs_algorithm algo, gs_dtype kdt, void* d_keys, gs_dtype vdt, void* d_vals,
+                        const uint64_t* h_rank_off, uint32_t p, uint32_t k_way, uint32_t samples_per_rank,
+                        int descending, gs_vrank_report* report);
+
+/* dst[i] = src[idx[i]] for a payload of elem_bytes (1, 2, 4, 8) bytes.
+ * Device form of ParUtils::ArrangeValues (include/par/par_utils.h:22-86). */
+gs_status gs_gather(gs_handle* h, int elem_bytes, const void* d_src, const uint64_t* d_idx, void* d_dst,
+                    uint64_t n);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* GPUSORT_HIP_H_ */

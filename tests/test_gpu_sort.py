@@ -1,0 +1,130 @@
+"""Parity of the device sort (gs_sort_keys / gs_sort_pairs) with the oracle.
+
+Keys-only results must equal the oracle's restatement of OmpUtils::MergeSort
+bit for bit; key-value results must equal the stable sort by key that
+thrust::sort_by_key gives (src/cuda_utils.cu:477), checked against both the
+oracle and numpy's stable argsort.
+"""
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+import oracle
+from gpusort_amd import datagen
+from tests.gpu_util import assert_bits_equal, dev, host, say, stable_order
+
+pytestmark = pytest.mark.gpu
+
+KEY_DTYPES = [np.uint32, np.int32, np.float32, np.uint64, np.int64, np.float64]
+
+# Sizes around the block-sort cap (8192 / 4096 / 2048) and the onesweep tile
+# (8192 / 6144 keys), plus multi-tile and ragged lengths.
+SIZES = [0, 1, 2, 3, 63, 64, 65, 1000, 2047, 2048, 2049, 4095, 4096, 4097, 6143, 6144, 6145, 8191, 8192, 8193,
+         12289, 16384, 100_003, 1 << 20]
+
+
+def make_keys(dtype, n: int, seed: int) -> np.ndarray:
+    d = np.dtype(dtype)
+    r = datagen.draws(seed, n)
+    if d == np.float32:
+        f = datagen.normal_f32(seed, n)
+        return f
+    if d == np.float64:
+        z = datagen.std_normal(seed, n) * 1e6
+        z[z == 0] = 0.0
+        return z.astype(np.float64)
+    if d.itemsize == 4:
+        return (r >> np.uint64(32)).astype(np.uint32).view(d)
+    return r.view(d)
+
+
+@pytest.mark.parametrize("dtype", KEY_DTYPES, ids=lambda d: np.dtype(d).name)
+@pytest.mark.parametrize("descending", [False, True], ids=["asc", "desc"])
+def test_sort_keys_sizes(handle, dtype, descending):
+    for n in SIZES:
+        keys = make_keys(dtype, n, seed=n + 7)
+        say(f"sort_keys dtype={np.dtype(dtype).name} n={n} desc={descending}")
+        t = dev(keys)
+        handle.sort_keys(t, descending=descending)
+        got = host(t)
+        want = oracle.merge_sort(keys, descending=descending)
+        assert_bits_equal(got, want, f"n={n}")
+
+
+@pytest.mark.parametrize("kdtype", KEY_DTYPES, ids=lambda d: np.dtype(d).name)
+@pytest.mark.parametrize("vdtype", [np.uint32, np.uint64], ids=["v32", "v64"])
+@pytest.mark.parametrize("descending", [False, True], ids=["asc", "desc"])
+def test_sort_pairs_stable(handle, kdtype, vdtype, descending):
+    for n in [0, 1, 5, 2048, 2049, 4097, 6145, 8193, 50_001, 1 << 19]:
+        # Few distinct keys so that stability is exercised.
+        keys = make_keys(kdtype, n, seed=3 * n + 1)
+        if n > 10:
+            keys = keys[datagen.draws(99, n) % np.uint64(max(1, n // 16))]
+        vals = np.arange(n, dtype=vdtype)
+        say(f"sort_pairs k={np.dtype(kdtype).name} v={np.dtype(vdtype).name} n={n} desc={descending}")
+        tk, tv = dev(keys), dev(vals)
+        handle.sort_pairs(tk, tv, descending=descending)
+        gk, gv = host(tk), host(tv)
+        wk, wv = oracle.stable_sort_pairs(keys, vals, descending=descending)
+        assert_bits_equal(gk, wk, f"keys n={n}")
+        assert_bits_equal(gv, wv, f"values n={n}")
+        perm = stable_order(keys, descending)
+        assert np.array_equal(gv, vals[perm])
+
+
+@pytest.mark.parametrize("dist", ["uniform", "zipf", "sorted", "equal"])
+def test_distributions_kv(handle, dist):
+    n = 3_000_017
+    keys = datagen.keys(dist, 11, n)
+    vals = np.arange(n, dtype=np.uint32)
+    say(f"distribution {dist} n={n}")
+    tk, tv = dev(keys), dev(vals)
+    handle.sort_pairs(tk, tv)
+    gk, gv = host(tk), host(tv)
+    perm = np.argsort(keys, kind="stable")
+    assert_bits_equal(gk, keys[perm], "keys")
+    assert_bits_equal(gv, perm.astype(np.uint32), "payload permutation")
+
+
+def test_float_special_order(handle):
+    # -0.0 orders before +0.0 under the order-preserving bit flip; the
+    # reference's std::less<float> leaves their relative order unspecified.
+    keys = np.array([3.5, -0.0, 0.0, -1e-30, 1e-30, -3.5, np.finfo(np.float32).max, -np.finfo(np.float32).max],
+                    dtype=np.float32)
+    t = dev(keys)
+    handle.sort_keys(t)
+    got = host(t)
+    assert got.tolist() == sorted(keys.tolist())
+    assert np.signbit(got[3]) and not np.signbit(got[4])
+
+
+def test_host_entries(handle):
+    n = 70_001
+    keys = datagen.uniform_u32(5, n)
+    want = oracle.merge_sort(keys)
+    k = keys.copy()
+    handle.sort_keys_host(k)
+    assert_bits_equal(k, want, "host keys")
+    keys = datagen.uniform_u64(6, n).view(np.int64)
+    vals = np.arange(n, dtype=np.uint32)
+    pairs = np.zeros(n, dtype=np.dtype([("k", np.int64), ("v", np.uint32), ("pad", np.uint32)]))
+    k, v = keys.copy(), vals.copy()
+    handle.sort_pairs_host(k, v, pairs_out=pairs, pair_stride=16, val_offset=8)
+    wk, wv = oracle.stable_sort_pairs(keys, vals)
+    assert_bits_equal(k, wk, "host pair keys")
+    assert_bits_equal(v, wv, "host pair values")
+    assert np.array_equal(pairs["k"], wk) and np.array_equal(pairs["v"], wv)
+
+
+def test_repeat_calls_reuse_arena(handle):
+    keys = datagen.uniform_u32(8, 1 << 20)
+    t = dev(keys)
+    handle.sort_keys(t)
+    before = handle.arena_bytes()
+    for s in range(3):
+        k2 = datagen.uniform_u32(100 + s, 1 << 20)
+        t = dev(k2)
+        handle.sort_keys(t)
+        assert_bits_equal(host(t), np.sort(k2), f"repeat {s}")
+    assert handle.arena_bytes() == before
