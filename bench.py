@@ -1,0 +1,277 @@
+#!/usr/bin/env python3
+"""Benchmark of the per-rank sort path on one MI355X.
+
+Workload (one "step"): a stable key-value sort of 2^28 uint32 keys with 2^28
+uint32 payloads (payload = original index), keys uniform over [0, 2^32) from
+a fixed seed, inputs resident in HBM before the timed region.  Each timed step
+sorts its own pre-generated copy of the input, so every step does the same
+work.  Prints ONE JSON line (rank 0, N=1).
+
+Also reported (fewer steps, same path): keys-only 2^28 u32, the Zipf /
+already-sorted / all-equal key distributions, and 2^27 float32 keys
+~ N(0, 1e3); the dominant kernel's HBM roofline fraction from event timing
+of every onesweep pass; and the CPU baseline (the reference's OmpUtils
+merge sort on the host cores, on a bounded sample of the same input).
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+if ROOT not in sys.path:
+    sys.path.insert(0, ROOT)
+
+METRIC = "Gkeys/s at 2^28 u32 (% HBM roofline); batched/served p50/p99 vs 1-call"
+HBM_PEAK_GBS = 8000.0
+SEED = 1
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def zipf_cdf(s: float = 1.1, distinct: int = 1 << 20) -> np.ndarray:
+    k = np.arange(1, distinct + 1, dtype=np.float64)
+    cdf = np.cumsum(np.power(k, -s))
+    cdf /= cdf[-1]
+    return cdf
+
+
+def make_input(h, g, torch, dist: str, n: int, seed: int):
+    """Device keys (and index payload) for a named distribution."""
+    if dist == "normal_f32":
+        keys = torch.empty(n, dtype=torch.float32, device="cuda")
+        h.generate(keys, g.GEN_NORMAL_F32, seed, param=1e3)
+    else:
+        keys = torch.empty(n, dtype=torch.uint32, device="cuda")
+        if dist == "uniform":
+            h.generate(keys, g.GEN_UNIFORM_U32, seed)
+        elif dist == "zipf":
+            h.generate(keys, g.GEN_ZIPF_U32, seed, cdf=zipf_cdf())
+        elif dist == "sorted":
+            h.generate(keys, g.GEN_SORTED_U32, seed, param=float(max(1, (1 << 32) // n)))
+        elif dist == "equal":
+            h.generate(keys, g.GEN_EQUAL_U32, seed)
+        else:
+            raise ValueError(dist)
+    vals = torch.empty(n, dtype=torch.uint32, device="cuda")
+    h.generate(vals, g.GEN_IOTA_U32, 0)
+    return keys, vals
+
+
+def as_i64(torch, t):
+    """Order-preserving widening for the checks (uint32 -> int64, float stays)."""
+    if t.dtype == torch.uint32:
+        return t.view(torch.int32).to(torch.int64) & 0xFFFFFFFF
+    return t
+
+
+def verify_kv(torch, k0, ks, vs) -> bool:
+    """Size-independent checks: sorted keys, payload is a permutation that maps
+    the input onto the output, and equal keys keep increasing payloads."""
+    kk = as_i64(torch, ks)
+    if not bool((kk[1:] >= kk[:-1]).all()):
+        return False
+    idx = as_i64(torch, vs)
+    if not bool((torch.bincount(idx, minlength=idx.numel()) == 1).all()):
+        return False
+    if not bool((as_i64(torch, k0)[idx] == kk).all()):
+        return False
+    eq = kk[1:] == kk[:-1]
+    return bool(((idx[1:] > idx[:-1]) | ~eq).all())
+
+
+def run_workload(h, g, torch, dist: str, n: int, steps: int, warmup: int, payload: bool, profile: bool):
+    keys0, vals0 = make_input(h, g, torch, dist, n, SEED)
+    torch.cuda.synchronize()
+    for _ in range(warmup):
+        k, v = keys0.clone(), vals0.clone()
+        if payload:
+            h.sort_pairs(k, v)
+        else:
+            h.sort_keys(k)
+    copies = [(keys0.clone(), vals0.clone() if payload else None) for _ in range(steps)]
+    torch.cuda.synchronize()
+    if profile:
+        h.profile_begin()
+    t0 = time.perf_counter()
+    for k, v in copies:
+        if payload:
+            h.sort_pairs(k, v)
+        else:
+            h.sort_keys(k)
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    prof = h.profile_end() if profile else {}
+    ok = True
+    k, v = copies[-1]
+    if payload:
+        ok = verify_kv(torch, keys0, k, v)
+    else:
+        kk = as_i64(torch, k)
+        ok = bool((kk[1:] >= kk[:-1]).all()) and int(kk.sum()) == int(as_i64(torch, keys0).sum())
+    del copies
+    torch.cuda.empty_cache()
+    ms = (t1 - t0) * 1e3 / steps
+    return {"gkeys_s": n / (ms * 1e-3) / 1e9, "ms_per_sort": ms, "verified": ok, "profile": prof}
+
+
+def roofline_of(prof: dict, name: str = "onesweep_pass"):
+    st = prof.get(name)
+    if not st or st["launches"] == 0:
+        return None
+    avg_ms = st["total_ms"] / st["launches"]
+    bytes_per = st["bytes"] / st["launches"]
+    achieved = bytes_per / (avg_ms * 1e-3) / 1e9
+    return {
+        "kernel": name,
+        "bound": "hbm",
+        "achieved": round(achieved, 1),
+        "peak": HBM_PEAK_GBS,
+        "unit": "GB/s",
+        "frac": round(achieved / HBM_PEAK_GBS, 4),
+        "traffic": None,
+        "bytes_per_launch": int(bytes_per),
+        "avg_launch_ms": round(avg_ms, 4),
+        "launches": st["launches"],
+    }
+
+
+def cpu_baseline(n_sample: int, reps: int = 3):
+    """The reference's OmpUtils::MergeSort on (key, index) pairs (the CPU local
+    sort of HykSort/SampleSort), on the first n_sample keys of the same input."""
+    import oracle
+    from gpusort_amd import datagen
+
+    keys = datagen.uniform_u32(SEED, n_sample)
+    vals = np.arange(n_sample, dtype=np.uint32)
+    kind = "reference" if oracle.ref_lib() is not None else "port"
+    fn = oracle.ref_merge_sort_pairs if kind == "reference" else oracle.merge_sort_pairs
+    times = []
+    out = None
+    for _ in range(reps):
+        t0 = time.perf_counter()
+        out = fn(keys, vals)
+        times.append(time.perf_counter() - t0)
+    perm = np.argsort(keys, kind="stable")
+    ok = bool(np.array_equal(out[1], perm.astype(np.uint32)))
+    t = float(np.median(times))
+    return {
+        "value": round(n_sample / t / 1e9, 4),
+        "unit": "Gkeys/s",
+        "cores": oracle.threads(),
+        "kind": kind,
+        "sample": f"first {n_sample} keys of the uniform u32 input with index payloads, "
+                  f"OmpUtils::MergeSort on std::pair, median of {reps}",
+        "seconds": round(t, 3),
+        "verified": ok,
+        "cpu_model": _cpu_model(),
+    }
+
+
+def _cpu_model() -> str:
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser(description=__doc__)
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--n", type=int, default=1 << 28, help="keys per sort")
+    ap.add_argument("--no-extra", action="store_true", help="skip the other distributions")
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument("--cpu-n", type=int, default=1 << 25, help="CPU baseline sample size")
+    args = ap.parse_args()
+
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU: --gpus {args.gpus} is not supported in this build "
+              f"(the per-rank path runs p virtual ranks on one device).", file=sys.stderr)
+        return 2
+
+    import torch
+
+    import gpusort_amd as g
+
+    if not torch.cuda.is_available():
+        print("no HIP device visible", file=sys.stderr)
+        return 1
+    h = g.Handle(0)
+    n = args.n
+
+    log(f"[bench] headline: KV u32+u32 uniform n={n} steps={args.steps} warmup={args.warmup}")
+    head = run_workload(h, g, torch, "uniform", n, args.steps, args.warmup, payload=True, profile=True)
+    roof = roofline_of(head["profile"])
+    hist = head["profile"].get("radix_hist")
+
+    extra = {}
+    if not args.no_extra:
+        log("[bench] keys-only u32 uniform")
+        r = run_workload(h, g, torch, "uniform", n, 3, 1, payload=False, profile=True)
+        extra["keys_only_u32_uniform"] = {"gkeys_s": round(r["gkeys_s"], 3), "ms": round(r["ms_per_sort"], 3),
+                                          "verified": r["verified"], "roofline": roofline_of(r["profile"])}
+        for dist in ["zipf", "sorted", "equal"]:
+            log(f"[bench] KV u32 {dist}")
+            r = run_workload(h, g, torch, dist, n, 3, 1, payload=True, profile=False)
+            extra[f"kv_u32_{dist}"] = {"gkeys_s": round(r["gkeys_s"], 3), "ms": round(r["ms_per_sort"], 3),
+                                       "verified": r["verified"]}
+        log("[bench] KV f32 normal 2^27")
+        r = run_workload(h, g, torch, "normal_f32", n // 2, 3, 1, payload=True, profile=False)
+        extra["kv_f32_normal_2^27"] = {"gkeys_s": round(r["gkeys_s"], 3), "ms": round(r["ms_per_sort"], 3),
+                                       "verified": r["verified"]}
+
+    cpu = None
+    if not args.no_cpu:
+        log(f"[bench] CPU baseline n={args.cpu_n}")
+        cpu = cpu_baseline(args.cpu_n)
+
+    out = {
+        "metric": METRIC,
+        "value": round(head["gkeys_s"], 3),
+        "unit": "Gkeys/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(head["ms_per_sort"], 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "u32",
+        "data": "synthetic",
+        "config": {
+            "workload": "plain large local sort: 2^28 uint32 keys + 2^28 uint32 payloads (original index), "
+                        "stable, uniform keys from seed 1",
+            "n_keys": n,
+            "key": "u32",
+            "payload": "u32",
+            "distribution": "uniform",
+        },
+        "verified": head["verified"],
+        "roofline": roof,
+        "hist_kernel": None if not hist else {
+            "avg_ms": round(hist["total_ms"] / hist["launches"], 4),
+            "GB/s": round(hist["bytes"] / hist["launches"] / (hist["total_ms"] / hist["launches"] * 1e-3) / 1e9, 1),
+        },
+        "cpu_baseline": cpu,
+        "extra": extra,
+    }
+    print(json.dumps(out), flush=True)
+    h.close()
+    return 0 if head["verified"] else 1
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -63,6 +63,21 @@ class VrankReport(ctypes.Structure):
     ]
 
 
+class KernelStat(ctypes.Structure):
+    _fields_ = [
+        ("name", ctypes.c_char * 48),
+        ("launches", ctypes.c_uint64),
+        ("total_ms", ctypes.c_double),
+        ("min_ms", ctypes.c_double),
+        ("max_ms", ctypes.c_double),
+        ("bytes", ctypes.c_uint64),
+    ]
+
+
+# gs_generate distribution codes
+GEN_UNIFORM_U32, GEN_UNIFORM_U64, GEN_ZIPF_U32, GEN_SORTED_U32, GEN_EQUAL_U32, GEN_NORMAL_F32, GEN_IOTA_U32, \
+    GEN_IOTA_U64 = range(8)
+
 _lib = None
 
 # Exported C symbols and their argument types.
@@ -108,6 +123,16 @@ _SIGS = {
     "gs_vrank_sort": (
         [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p,
          ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_int, ctypes.POINTER(VrankReport)],
+        ctypes.c_int,
+    ),
+    "gs_generate": (
+        [ctypes.c_void_p, ctypes.c_int, ctypes.c_uint64, ctypes.c_void_p, ctypes.c_uint64, ctypes.c_uint64,
+         ctypes.c_double, ctypes.c_void_p, ctypes.c_uint32],
+        ctypes.c_int,
+    ),
+    "gs_profile_begin": ([ctypes.c_void_p], ctypes.c_int),
+    "gs_profile_end": (
+        [ctypes.c_void_p, ctypes.POINTER(KernelStat), ctypes.c_uint32, ctypes.POINTER(ctypes.c_uint32)],
         ctypes.c_int,
     ),
     "gs_gather": (
@@ -305,6 +330,33 @@ class Handle:
             "final_rank_sizes": fin.copy(),
             "max_rank_error": rep.max_rank_error,
         }
+
+    def generate(self, out, dist: int, seed: int, index_base: int = 0, param: float = 0.0, cdf=None) -> None:
+        """Fills a device tensor with a seeded synthetic distribution (gs_generate)."""
+        cdf_arr = None if cdf is None else np.ascontiguousarray(cdf, dtype=np.float64)
+        _check(
+            self.lib.gs_generate(
+                self.h, dist, seed & 0xFFFFFFFFFFFFFFFF, _dev_ptr(out), out.numel(), index_base, float(param),
+                None if cdf_arr is None else cdf_arr.ctypes.data, 0 if cdf_arr is None else cdf_arr.size,
+            )
+        )
+
+    def profile_begin(self) -> None:
+        _check(self.lib.gs_profile_begin(self.h))
+
+    def profile_end(self) -> dict:
+        """Per-kernel {name: {launches, total_ms, min_ms, max_ms, bytes}} since profile_begin."""
+        arr = (KernelStat * 64)()
+        n = ctypes.c_uint32(0)
+        _check(self.lib.gs_profile_end(self.h, arr, 64, ctypes.byref(n)))
+        out = {}
+        for i in range(min(n.value, 64)):
+            st = arr[i]
+            out[st.name.decode()] = {
+                "launches": st.launches, "total_ms": st.total_ms, "min_ms": st.min_ms, "max_ms": st.max_ms,
+                "bytes": st.bytes,
+            }
+        return out
 
     def gather(self, src, idx, dst) -> None:
         _check(self.lib.gs_gather(self.h, src.element_size(), _dev_ptr(src), _dev_ptr(idx), _dev_ptr(dst), dst.numel()))

@@ -90,6 +90,38 @@ void Arena::release() {
   }
 }
 
+// ---------------------------------------------------------------- profiler
+Profiler::~Profiler() {
+  for (hipEvent_t e : pool_) (void)hipEventDestroy(e);
+}
+
+hipEvent_t Profiler::take() {
+  if (used_ == pool_.size()) {
+    hipEvent_t e = nullptr;
+    if (hipEventCreate(&e) != hipSuccess) return nullptr;
+    pool_.push_back(e);
+  }
+  return pool_[used_++];
+}
+
+void Profiler::begin(const char* name, uint64_t bytes, hipStream_t stream) {
+  Rec r{name, bytes, take(), nullptr};
+  if (r.start) (void)hipEventRecord(r.start, stream);
+  recs.push_back(r);
+}
+
+void Profiler::end(hipStream_t stream) {
+  if (recs.empty()) return;
+  Rec& r = recs.back();
+  r.stop = take();
+  if (r.stop) (void)hipEventRecord(r.stop, stream);
+}
+
+void Profiler::reset() {
+  recs.clear();
+  used_ = 0;
+}
+
 DeviceScope::DeviceScope(int dev) {
   if (hipGetDevice(&prev_) != hipSuccess) prev_ = -1;
   if (prev_ != dev) (void)hipSetDevice(dev);
@@ -117,7 +149,7 @@ gs_status sort_device(gs_handle* h, gs_dtype kdt, void* d_keys, gs_dtype vdt, vo
   void* ws = nullptr;
   hipError_t e = h->arena.get(Arena::kRadix, ws_bytes, h->stream, &ws);
   if (e != hipSuccess) return from_hip(e, "radix workspace");
-  return from_hip(radix_sort(kb, vb, d_keys, d_vals, n, xf, ws, ws_bytes, h->stream), "radix sort");
+  return from_hip(radix_sort(kb, vb, d_keys, d_vals, n, xf, ws, ws_bytes, h->stream, h->timer()), "radix sort");
 }
 
 }  // namespace gs
@@ -347,6 +379,47 @@ gs_status gs_bucket_bounds(gs_handle* h, gs_dtype kdt, const void* d_keys, const
   if (e == hipSuccess) e = hipMemcpyAsync(h_bounds, w + o_b, n_b * 8, hipMemcpyDeviceToHost, h->stream);
   if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
   return from_hip(e, "bucket bounds");
+}
+
+gs_status gs_profile_begin(gs_handle* h) {
+  GS_CHECK_HANDLE(h);
+  DeviceScope ds(h->device);
+  h->profiler.reset();
+  h->profiler.active = true;
+  return GS_OK;
+}
+
+gs_status gs_profile_end(gs_handle* h, gs_kernel_stat* out, uint32_t max_out, uint32_t* n_out) {
+  GS_CHECK_HANDLE(h);
+  if (!n_out || (max_out && !out)) return fail(GS_INVALID_ARG, "null output");
+  DeviceScope ds(h->device);
+  h->profiler.active = false;
+  hipError_t e = hipStreamSynchronize(h->stream);
+  if (e != hipSuccess) return from_hip(e, "profile synchronize");
+  std::vector<gs_kernel_stat> agg;
+  for (const auto& r : h->profiler.recs) {
+    float ms = 0.f;
+    if (!r.start || !r.stop || hipEventElapsedTime(&ms, r.start, r.stop) != hipSuccess) continue;
+    gs_kernel_stat* st = nullptr;
+    for (auto& a : agg)
+      if (r.name == a.name) st = &a;
+    if (!st) {
+      gs_kernel_stat z{};
+      snprintf(z.name, sizeof(z.name), "%s", r.name.c_str());
+      z.min_ms = 1e30;
+      agg.push_back(z);
+      st = &agg.back();
+    }
+    st->launches += 1;
+    st->total_ms += ms;
+    st->bytes += r.bytes;
+    if (ms > st->max_ms) st->max_ms = ms;
+    if (ms < st->min_ms) st->min_ms = ms;
+  }
+  h->profiler.reset();
+  *n_out = (uint32_t)agg.size();
+  for (uint32_t i = 0; i < agg.size() && i < max_out; ++i) out[i] = agg[i];
+  return GS_OK;
 }
 
 gs_status gs_gather(gs_handle* h, int elem_bytes, const void* d_src, const uint64_t* d_idx, void* d_dst, uint64_t n) {

@@ -6,9 +6,34 @@
 #include <stddef.h>
 #include <stdint.h>
 
+#include <string>
+#include <vector>
+
 #include "../../include/gpusort_hip.h"
+#include "gs_internal.h"
 
 namespace gs {
+
+// Event-pair timing of kernel launches (gs_profile_begin / gs_profile_end).
+class Profiler : public LaunchTimer {
+ public:
+  struct Rec {
+    std::string name;
+    uint64_t bytes;
+    hipEvent_t start, stop;
+  };
+  ~Profiler() override;
+  void begin(const char* name, uint64_t bytes, hipStream_t stream) override;
+  void end(hipStream_t stream) override;
+  void reset();
+  std::vector<Rec> recs;
+  bool active = false;
+
+ private:
+  std::vector<hipEvent_t> pool_;
+  size_t used_ = 0;
+  hipEvent_t take();
+};
 
 // Grow-only device buffers, one per use.  A slot is reallocated only when a
 // call needs more bytes than it holds (after the stream drains); steady-state
@@ -57,6 +82,8 @@ struct gs_handle {
   int device = 0;
   hipStream_t stream = nullptr;
   gs::Arena arena;
+  gs::Profiler profiler;
+  gs::LaunchTimer* timer() { return profiler.active ? &profiler : nullptr; }
   void* pinned = nullptr;
   size_t pinned_bytes = 0;
 };

@@ -79,13 +79,21 @@ __device__ inline uint32_t count_below(uint64_t m) {
 // is the set of lanes that differ in at least one bit, and then inverted.
 template <int BITS>
 __device__ inline uint64_t match_any(uint32_t label) {
+  // Sign-extended bit b (all ones if set): one v_bfe_i32 per bit.  All
+  // ballots are taken before they are consumed, and each half of the mask
+  // folds in with one 3-input bit op: diff | (ballot ^ own) is truth table
+  // 0xF6 with operands (diff, ballot, own).
+  uint32_t own[BITS];
+  uint64_t bal[BITS];
+#pragma unroll
+  for (int b = 0; b < BITS; ++b) own[b] = (uint32_t)((int32_t)(label << (31 - b)) >> 31);
+#pragma unroll
+  for (int b = 0; b < BITS; ++b) bal[b] = ballot((int32_t)own[b] < 0);
   uint32_t diff_lo = 0, diff_hi = 0;
 #pragma unroll
   for (int b = 0; b < BITS; ++b) {
-    const uint32_t own = 0u - ((label >> b) & 1u);  // all ones if bit set
-    const uint64_t bal = ballot(own != 0u);
-    diff_lo |= (uint32_t)bal ^ own;
-    diff_hi |= (uint32_t)(bal >> 32) ^ own;
+    diff_lo = __builtin_amdgcn_bitop3_b32(diff_lo, (uint32_t)bal[b], own[b], 0xF6);
+    diff_hi = __builtin_amdgcn_bitop3_b32(diff_hi, (uint32_t)(bal[b] >> 32), own[b], 0xF6);
   }
   return ~(((uint64_t)diff_hi << 32) | diff_lo);
 }
@@ -129,6 +137,25 @@ __device__ inline S status_load(const S* p) {
 template <typename S>
 __device__ inline void status_store(S* p, S v) {
   __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+// Cache-allowed read of a status word.  A status word only ever moves
+// 0 -> aggregate -> inclusive prefix, and every non-zero value it has held is
+// correct, so a stale cached copy is usable unless it is 0; callers re-read a
+// 0 with status_load.  Workgroup scope keeps it an atomic access (never
+// hoisted out of a polling loop) that the L1/L2 may serve.
+template <typename S>
+__device__ inline S status_load_cached(const S* p) {
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+}
+
+// Workgroup barrier that orders LDS only.  __syncthreads also waits for every
+// outstanding global store of the calling wave to complete; the sort kernels
+// share nothing through global memory inside a workgroup, so their barriers
+// skip that wait and store latency overlaps the next phase.
+__device__ inline void lds_barrier() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+  __builtin_amdgcn_s_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
 }
 
 // Fill kernel used instead of hipMemsetAsync so that every clear is a kernel

@@ -40,13 +40,23 @@ struct SortCtrl {
   uint32_t pad[14];
 };
 
+// Optional per-launch timing: when a launcher is given a timer it brackets
+// each kernel with events recorded on the launch stream.  `bytes` is the
+// algorithmic HBM traffic of the launch (inputs read + outputs written once).
+class LaunchTimer {
+ public:
+  virtual ~LaunchTimer() {}
+  virtual void begin(const char* name, uint64_t bytes, hipStream_t stream) = 0;
+  virtual void end(hipStream_t stream) = 0;
+};
+
 // ---------------------------------------------------------------- radix sort
 // Onesweep LSD radix sort of n keys (key_bytes 4/8) with an optional payload
 // (val_bytes 0/4/8), stable.  keys/vals are sorted in place; the workspace
 // holds the ping-pong buffers, histograms and look-back status words.
 size_t radix_workspace_bytes(int key_bytes, int val_bytes, uint64_t n);
 hipError_t radix_sort(int key_bytes, int val_bytes, void* keys, void* vals, uint64_t n, const XformParams& xf,
-                      void* workspace, size_t workspace_bytes, hipStream_t stream);
+                      void* workspace, size_t workspace_bytes, hipStream_t stream, LaunchTimer* timer = nullptr);
 
 // Tile geometry of the onesweep pass for a key/value width pair.
 uint32_t radix_tile_keys(int key_bytes, int val_bytes);

@@ -33,6 +33,8 @@
 namespace gs {
 
 constexpr int kRadix = 256;
+// Predecessor status words read per look-back step.
+constexpr int kLookback = 8;
 
 template <typename U>
 __device__ inline void hist_add(uint32_t* h, uint32_t d) {
@@ -150,10 +152,21 @@ struct OnesweepArgs {
   const uint64_t* digit_base;  // 256 offsets of this pass
   S* status;                   // tiles*256 status words of this pass, zeroed
   SortCtrl* ctrl;
+  uint32_t tiles;
 };
 
-template <typename U, typename V, bool KV, int THREADS, int ITEMS, typename S>
+// Digit of a key: the ordered image's byte at `shift`.  IDENT is the
+// unsigned-ascending case where the image is the key itself.
+template <typename U, bool IDENT>
+__device__ inline uint32_t key_digit(U key, const Xform<U>& xf, int shift) {
+  if constexpr (IDENT) return digit_of(key, shift);
+  return digit_of(to_ordered(key, xf), shift);
+}
+
+template <typename U, typename V, bool KV, int THREADS, int ITEMS, typename S, bool IDENT>
 __global__ __launch_bounds__(THREADS) void onesweep_kernel(OnesweepArgs<U, V, S> a) {
+  // Positions fit 32 bits whenever the 32-bit status word is used (n < 2^30).
+  using Idx = typename std::conditional<sizeof(S) == 4, uint32_t, uint64_t>::type;
   constexpr int WAVES = THREADS / kWave;
   constexpr uint32_t TILE = THREADS * ITEMS;
   constexpr int FLAG_SHIFT = sizeof(S) * 8 - 2;
@@ -162,9 +175,13 @@ __global__ __launch_bounds__(THREADS) void onesweep_kernel(OnesweepArgs<U, V, S>
   constexpr S VAL_MASK = FLAG_AGG - 1;
   constexpr size_t BUF_ELEM = (sizeof(U) > sizeof(V) || !KV) ? sizeof(U) : sizeof(V);
 
+  // s_wave_hist[w][d]: wave w's count of digit d during ranking, then the
+  // tile-local slot of wave w's first key with digit d.
   __shared__ uint32_t s_wave_hist[WAVES * kRadix];
-  __shared__ uint32_t s_local_start[kRadix];
-  __shared__ uint64_t s_gofs[kRadix];
+  // s_match[w][d]: lane mask of wave w's lanes holding digit d for the item
+  // being ranked; every entry is back to 0 after each item.
+  __shared__ unsigned long long s_match[WAVES * kRadix];
+  __shared__ Idx s_gofs[kRadix];
   __shared__ uint32_t s_wsum[4];
   __shared__ uint32_t s_tile;
   __shared__ __attribute__((aligned(16))) unsigned char s_buf[TILE * BUF_ELEM];
@@ -175,155 +192,181 @@ __global__ __launch_bounds__(THREADS) void onesweep_kernel(OnesweepArgs<U, V, S>
   U* __restrict__ kout = src ? a.keys0 : a.keys1;
   const V* __restrict__ vin = src ? a.vals1 : a.vals0;
   V* __restrict__ vout = src ? a.vals0 : a.vals1;
+  const int shift = a.shift;
 
   const uint32_t tid = threadIdx.x;
   const uint32_t lane = tid & 63u;
   const uint32_t wave = tid >> 6;
 
   if (tid == 0) s_tile = atomicAdd(&a.ctrl->tile_counter[a.pass], 1u);
-  for (uint32_t i = tid; i < WAVES * kRadix; i += THREADS) s_wave_hist[i] = 0;
-  __syncthreads();
+  for (uint32_t i = tid; i < WAVES * kRadix; i += THREADS) {
+    s_wave_hist[i] = 0;
+    s_match[i] = 0;
+  }
+  lds_barrier();
   const uint32_t tile = s_tile;
   const uint64_t tile_base = (uint64_t)tile * TILE;
   const uint64_t remain = a.n - tile_base;
   const uint32_t valid = remain < TILE ? (uint32_t)remain : TILE;
+  const bool full = valid == TILE;
 
   // 1. wave-striped loads: item i of lane l of wave w is key
   //    tile_base + w*64*ITEMS + i*64 + l, so item order within a wave is key order.
+  const U* __restrict__ tk = kin + tile_base;
+  const V* __restrict__ tv = vin + tile_base;
+  const uint32_t lofs = wave * (kWave * ITEMS) + lane;
   U k[ITEMS];
   V v[KV ? ITEMS : 1];
-  uint32_t r[ITEMS];
-  const uint64_t wbase = tile_base + (uint64_t)wave * (kWave * ITEMS) + lane;
-  if (valid == TILE) {
+  if (full) {
 #pragma unroll
-    for (int i = 0; i < ITEMS; ++i) k[i] = kin[wbase + (uint64_t)i * kWave];
+    for (int i = 0; i < ITEMS; ++i) k[i] = tk[lofs + i * kWave];
     if constexpr (KV) {
 #pragma unroll
-      for (int i = 0; i < ITEMS; ++i) v[i] = vin[wbase + (uint64_t)i * kWave];
+      for (int i = 0; i < ITEMS; ++i) v[i] = tv[lofs + i * kWave];
     }
   } else {
 #pragma unroll
-    for (int i = 0; i < ITEMS; ++i) {
-      const uint64_t idx = wbase + (uint64_t)i * kWave;
-      k[i] = idx < a.n ? kin[idx] : U(0);
-    }
+    for (int i = 0; i < ITEMS; ++i) k[i] = (lofs + i * kWave) < valid ? tk[lofs + i * kWave] : U(0);
     if constexpr (KV) {
 #pragma unroll
-      for (int i = 0; i < ITEMS; ++i) {
-        const uint64_t idx = wbase + (uint64_t)i * kWave;
-        v[i] = idx < a.n ? vin[idx] : V(0);
-      }
+      for (int i = 0; i < ITEMS; ++i) v[i] = (lofs + i * kWave) < valid ? tv[lofs + i * kWave] : V(0);
     }
   }
 
   // Keys past the end of the array take digit 255: they are the last keys of
   // the tile in key order, so they land on the last (TILE - valid) slots.
-  const bool full = valid == TILE;
   auto digit_at = [&](int i) -> uint32_t {
-    const uint32_t d = digit_of(to_ordered(k[i], a.xf), a.shift);
+    const uint32_t d = key_digit<U, IDENT>(k[i], a.xf, shift);
     if (full) return d;
-    return (wbase + (uint64_t)i * kWave) < a.n ? d : 255u;
+    return (lofs + i * kWave) < valid ? d : 255u;
   };
 
-  // 2. stable rank inside the wave.
+  // 2. stable rank inside the wave.  Lanes holding the same digit are found
+  //    through LDS: each lane ORs its lane bit into its digit's mask and reads
+  //    the mask back, then the mask is cleared; every matching lane then
+  //    writes the same new digit count, so no lane election is needed.
   uint32_t* wh = s_wave_hist + wave * kRadix;
+  unsigned long long* wm = s_match + wave * kRadix;
+  uint32_t r[ITEMS];
 #pragma unroll
   for (int i = 0; i < ITEMS; ++i) {
     const uint32_t d = digit_at(i);
-    const uint64_t peers = match_any<8>(d);
+    __hip_atomic_fetch_or(&wm[d], 1ull << lane, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+    __builtin_amdgcn_wave_barrier();
+    const uint64_t peers = __hip_atomic_load(&wm[d], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+    __builtin_amdgcn_wave_barrier();
+    __hip_atomic_store(&wm[d], 0ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
     const uint32_t below = count_below(peers);
     const uint32_t old = wh[d];
     __builtin_amdgcn_wave_barrier();
-    if (below == 0) wh[d] = old + (uint32_t)__popcll(peers);
+    wh[d] = old + (uint32_t)__popcll(peers);
     __builtin_amdgcn_wave_barrier();
     r[i] = old + below;
   }
-  __syncthreads();
+  lds_barrier();
 
-  // 3. cross-wave offsets per digit, tile digit totals, publish aggregate.
-  uint32_t total = 0, pub = 0;
+  // 3. per digit: cross-wave totals (published at once), block scan of the
+  //    totals, then the tile-local start of every digit folded into the
+  //    per-wave offsets.
+  uint32_t total = 0, pub = 0, incl = 0;
   if (tid < kRadix) {
 #pragma unroll
-    for (int w = 0; w < WAVES; ++w) {
-      const uint32_t c = s_wave_hist[w * kRadix + tid];
-      s_wave_hist[w * kRadix + tid] = total;
-      total += c;
-    }
+    for (int w = 0; w < WAVES; ++w) total += s_wave_hist[w * kRadix + tid];
     pub = total - ((tid == kRadix - 1) ? (TILE - valid) : 0u);
     const S word = (tile == 0) ? (FLAG_PRE | (S)(a.digit_base[tid] + pub)) : (FLAG_AGG | (S)pub);
     status_store(&a.status[(uint64_t)tile * kRadix + tid], word);
-  }
-  uint32_t incl = 0;
-  if (tid < kRadix) {
     incl = wave_incl_scan(total);
     if (lane == 63) s_wsum[wave] = incl;
   }
-  __syncthreads();
+  lds_barrier();
+  uint32_t local_start = 0;
   if (tid < kRadix) {
     uint32_t add = 0;
     for (uint32_t w = 0; w < wave; ++w) add += s_wsum[w];
-    s_local_start[tid] = incl - total + add;
+    local_start = incl - total + add;
+    uint32_t run = local_start;
+#pragma unroll
+    for (int w = 0; w < WAVES; ++w) {
+      const uint32_t c = s_wave_hist[w * kRadix + tid];
+      s_wave_hist[w * kRadix + tid] = run;
+      run += c;
+    }
   }
-  __syncthreads();
+  lds_barrier();
 
   // 4. local slots; tile sorted by digit in LDS.
   U* s_k = reinterpret_cast<U*>(s_buf);
 #pragma unroll
   for (int i = 0; i < ITEMS; ++i) {
-    const uint32_t d = digit_at(i);
-    r[i] += s_local_start[d] + s_wave_hist[wave * kRadix + d];
+    r[i] += wh[digit_at(i)];
     s_k[r[i]] = k[i];
   }
 
-  // 5. decoupled look-back, one lane per digit.
+  // 5. decoupled look-back, one lane per digit.  Each step reads a window of
+  //    kLookback predecessor words at once, then consumes them newest first:
+  //    aggregates are added until an inclusive prefix closes the sum; a word
+  //    not yet published ends the step and is polled again.
   if (tid < kRadix) {
+    constexpr int W = kLookback;
     uint64_t excl;
     if (tile == 0) {
       excl = a.digit_base[tid];
     } else {
       excl = 0;
-      int64_t t = (int64_t)tile - 1;
-      for (;;) {
-        const S w = status_load(&a.status[(uint64_t)t * kRadix + tid]);
-        const S flag = w & ~VAL_MASK;
-        if (flag == 0) {
-          __builtin_amdgcn_s_sleep(1);
-          continue;
+      int64_t t = (int64_t)tile - 1;  // newest predecessor not yet consumed
+      bool done = false;
+      while (!done) {
+        S w[W];
+#pragma unroll
+        for (int j = 0; j < W; ++j)
+          w[j] = (t - j >= 0) ? status_load_cached(&a.status[(uint64_t)(t - j) * kRadix + tid]) : S(0);
+        // A 0 may be a stale cached copy: the newest unconsumed word is
+        // re-read device-coherently, so an unpublished word is always polled
+        // from memory.
+        if ((w[0] & ~VAL_MASK) == 0) w[0] = status_load(&a.status[(uint64_t)t * kRadix + tid]);
+        int used = 0;
+#pragma unroll
+        for (int j = 0; j < W; ++j) {
+          if (done || used < j) continue;
+          const S flag = w[j] & ~VAL_MASK;
+          if (flag == 0) continue;
+          excl += (uint64_t)(w[j] & VAL_MASK);
+          used = j + 1;
+          if (flag == FLAG_PRE) done = true;
         }
-        excl += (uint64_t)(w & VAL_MASK);
-        if (flag == FLAG_PRE) break;
-        --t;
+        t -= used;
+        if (!done && used < W) __builtin_amdgcn_s_sleep(1);
       }
       status_store(&a.status[(uint64_t)tile * kRadix + tid], (S)(FLAG_PRE | (S)(excl + pub)));
     }
-    s_gofs[tid] = excl - (uint64_t)s_local_start[tid];
+    s_gofs[tid] = (Idx)(excl - local_start);
   }
-  __syncthreads();
+  lds_barrier();
 
   // 6. coalesced scatter of keys in slot order.
-  uint32_t dig[KV ? ITEMS : 1];
+  Idx g[KV ? ITEMS : 1];
 #pragma unroll
   for (int j = 0; j < ITEMS; ++j) {
     const uint32_t p = (uint32_t)j * THREADS + tid;
     if (p < valid) {
       const U key = s_k[p];
-      const uint32_t d = digit_of(to_ordered(key, a.xf), a.shift);
-      kout[s_gofs[d] + p] = key;
-      if constexpr (KV) dig[j] = d;
+      const Idx dst = s_gofs[key_digit<U, IDENT>(key, a.xf, shift)] + (Idx)p;
+      kout[dst] = key;
+      if constexpr (KV) g[j] = dst;
     }
   }
 
   // 7. payloads through the same slots.
   if constexpr (KV) {
-    __syncthreads();
+    lds_barrier();
     V* s_v = reinterpret_cast<V*>(s_buf);
 #pragma unroll
     for (int i = 0; i < ITEMS; ++i) s_v[r[i]] = v[i];
-    __syncthreads();
+    lds_barrier();
 #pragma unroll
     for (int j = 0; j < ITEMS; ++j) {
       const uint32_t p = (uint32_t)j * THREADS + tid;
-      if (p < valid) vout[s_gofs[dig[j]] + p] = s_v[p];
+      if (p < valid) vout[g[j]] = s_v[p];
     }
   }
 }
@@ -348,11 +391,11 @@ struct Geometry;
 // and payload, reused for the payload pass.
 template <>
 struct Geometry<uint32_t, 0> {
-  static constexpr int THREADS = 512, ITEMS = 16;
+  static constexpr int THREADS = 1024, ITEMS = 16;
 };
 template <>
 struct Geometry<uint32_t, 4> {
-  static constexpr int THREADS = 512, ITEMS = 16;
+  static constexpr int THREADS = 1024, ITEMS = 16;
 };
 template <>
 struct Geometry<uint32_t, 8> {
@@ -434,12 +477,13 @@ static Xform<U> xform_of(const XformParams& p) {
 
 template <typename U, int VB, typename S>
 static hipError_t radix_sort_t(U* keys, void* vals_v, uint64_t n, const XformParams& xp, unsigned char* ws,
-                               const RadixLayout& L, hipStream_t stream) {
+                               const RadixLayout& L, hipStream_t stream, LaunchTimer* timer) {
   using V = typename ValT<VB>::T;
   constexpr bool KV = VB != 0;
   constexpr int THREADS = Geometry<U, VB>::THREADS;
   constexpr int ITEMS = Geometry<U, VB>::ITEMS;
   const Xform<U> xf = xform_of<U>(xp);
+  const bool ident = xf.float_mask == 0 && xf.sign_flip == 0 && xf.desc_mask == 0;
   U* keys_tmp = reinterpret_cast<U*>(ws + L.off_keys_tmp);
   V* vals = reinterpret_cast<V*>(vals_v);
   V* vals_tmp = reinterpret_cast<V*>(ws + L.off_vals_tmp);
@@ -456,8 +500,10 @@ static hipError_t radix_sort_t(U* keys, void* vals_v, uint64_t n, const XformPar
   uint64_t hist_blocks = (nvec + 256 * 8 - 1) / (256 * 8);
   if (hist_blocks < 1) hist_blocks = 1;
   if (hist_blocks > 2048) hist_blocks = 2048;
+  if (timer) timer->begin("radix_hist", n * sizeof(U) + status_bytes, stream);
   hipLaunchKernelGGL(radix_hist_kernel<U>, dim3((uint32_t)hist_blocks), dim3(256), 0, stream, keys, n, xf,
                      L.passes, hist, reinterpret_cast<uint4*>(status), status_bytes / 16);
+  if (timer) timer->end(stream);
   hipLaunchKernelGGL(radix_scan_kernel, dim3(1), dim3(256), 0, stream, hist, base, ctrl, L.passes, n);
   for (int p = 0; p < L.passes; ++p) {
     OnesweepArgs<U, V, S> a;
@@ -472,8 +518,15 @@ static hipError_t radix_sort_t(U* keys, void* vals_v, uint64_t n, const XformPar
     a.digit_base = base + p * kRadix;
     a.status = status + p * status_per_pass;
     a.ctrl = ctrl;
-    hipLaunchKernelGGL((onesweep_kernel<U, V, KV, THREADS, ITEMS, S>), dim3((uint32_t)L.tiles), dim3(THREADS), 0,
-                       stream, a);
+    a.tiles = (uint32_t)L.tiles;
+    if (timer) timer->begin("onesweep_pass", 2 * n * (sizeof(U) + (KV ? sizeof(V) : 0)), stream);
+    if (ident)
+      hipLaunchKernelGGL((onesweep_kernel<U, V, KV, THREADS, ITEMS, S, true>), dim3((uint32_t)L.tiles),
+                         dim3(THREADS), 0, stream, a);
+    else
+      hipLaunchKernelGGL((onesweep_kernel<U, V, KV, THREADS, ITEMS, S, false>), dim3((uint32_t)L.tiles),
+                         dim3(THREADS), 0, stream, a);
+    if (timer) timer->end(stream);
   }
   uint64_t copy_blocks = (n + 255) / 256;
   if (copy_blocks > 4096) copy_blocks = 4096;
@@ -484,27 +537,27 @@ static hipError_t radix_sort_t(U* keys, void* vals_v, uint64_t n, const XformPar
 
 template <typename U, int VB>
 static hipError_t radix_sort_s(U* keys, void* vals, uint64_t n, const XformParams& xp, unsigned char* ws,
-                               const RadixLayout& L, hipStream_t stream) {
-  if (L.status_word == 4) return radix_sort_t<U, VB, uint32_t>(keys, vals, n, xp, ws, L, stream);
-  return radix_sort_t<U, VB, uint64_t>(keys, vals, n, xp, ws, L, stream);
+                               const RadixLayout& L, hipStream_t stream, LaunchTimer* timer) {
+  if (L.status_word == 4) return radix_sort_t<U, VB, uint32_t>(keys, vals, n, xp, ws, L, stream, timer);
+  return radix_sort_t<U, VB, uint64_t>(keys, vals, n, xp, ws, L, stream, timer);
 }
 
 hipError_t radix_sort(int kb, int vb, void* keys, void* vals, uint64_t n, const XformParams& xf, void* workspace,
-                      size_t workspace_bytes, hipStream_t stream) {
+                      size_t workspace_bytes, hipStream_t stream, LaunchTimer* timer) {
   if (n == 0) return hipSuccess;
   const RadixLayout L = radix_layout(kb, vb, n);
   if (workspace_bytes < L.total) return hipErrorInvalidValue;
   unsigned char* ws = reinterpret_cast<unsigned char*>(workspace);
   if (kb == 4) {
     uint32_t* k = reinterpret_cast<uint32_t*>(keys);
-    if (vb == 0) return radix_sort_s<uint32_t, 0>(k, vals, n, xf, ws, L, stream);
-    if (vb == 4) return radix_sort_s<uint32_t, 4>(k, vals, n, xf, ws, L, stream);
-    if (vb == 8) return radix_sort_s<uint32_t, 8>(k, vals, n, xf, ws, L, stream);
+    if (vb == 0) return radix_sort_s<uint32_t, 0>(k, vals, n, xf, ws, L, stream, timer);
+    if (vb == 4) return radix_sort_s<uint32_t, 4>(k, vals, n, xf, ws, L, stream, timer);
+    if (vb == 8) return radix_sort_s<uint32_t, 8>(k, vals, n, xf, ws, L, stream, timer);
   } else if (kb == 8) {
     uint64_t* k = reinterpret_cast<uint64_t*>(keys);
-    if (vb == 0) return radix_sort_s<uint64_t, 0>(k, vals, n, xf, ws, L, stream);
-    if (vb == 4) return radix_sort_s<uint64_t, 4>(k, vals, n, xf, ws, L, stream);
-    if (vb == 8) return radix_sort_s<uint64_t, 8>(k, vals, n, xf, ws, L, stream);
+    if (vb == 0) return radix_sort_s<uint64_t, 0>(k, vals, n, xf, ws, L, stream, timer);
+    if (vb == 4) return radix_sort_s<uint64_t, 4>(k, vals, n, xf, ws, L, stream, timer);
+    if (vb == 8) return radix_sort_s<uint64_t, 8>(k, vals, n, xf, ws, L, stream, timer);
   }
   return hipErrorInvalidValue;
 }

@@ -159,6 +159,36 @@ gs_status gs_vrank_sort(gs_handle* h, gs_algorithm algo, gs_dtype kdt, void* d_k
                         const uint64_t* h_rank_off, uint32_t p, uint32_t k_way, uint32_t samples_per_rank,
                         int descending, gs_vrank_report* report);
 
+/* Seeded synthetic keys written to device memory: draw i of stream `seed` is
+ * splitmix64(seed + (i + index_base + 1) * 0x9E3779B97F4A7C15).
+ * dist: 0 uniform u32 (high half), 1 uniform u64, 2 Zipf u32 (inverse CDF
+ * over the host table h_cdf of cdf_n ranks, rank mixed by a 32-bit
+ * bijection), 3 sorted u32 (i*param + draw % param), 4 all-equal u32,
+ * 5 normal f32 (Box-Muller, sigma = param, -0.0 -> +0.0), 6 u32 index,
+ * 7 u64 index.  No reference counterpart in the library: the reference
+ * generates test data on the host in its drivers (src/test/utils.h:425-444,
+ * :586-595). */
+gs_status gs_generate(gs_handle* h, int dist, uint64_t seed, void* d_out, uint64_t n, uint64_t index_base,
+                      double param, const double* h_cdf, uint32_t cdf_n);
+
+/* Kernel timing: between gs_profile_begin and gs_profile_end every kernel the
+ * handle launches for a sort is bracketed by events on its stream;
+ * gs_profile_end synchronises and returns per-kernel aggregates.  Bytes are
+ * the algorithmic HBM bytes of the launches (one read of the inputs, one
+ * write of the outputs).  Counterpart of the reference's phase timers
+ * (src/cuda_utils.cu:29-34, src/timer.cc:24-32). */
+typedef struct gs_kernel_stat {
+  char name[48];
+  uint64_t launches;
+  double total_ms;
+  double min_ms;
+  double max_ms;
+  uint64_t bytes;
+} gs_kernel_stat;
+
+gs_status gs_profile_begin(gs_handle* h);
+gs_status gs_profile_end(gs_handle* h, gs_kernel_stat* out, uint32_t max_out, uint32_t* n_out);
+
 /* dst[i] = src[idx[i]] for a payload of elem_bytes (1, 2, 4, 8) bytes.
  * Device form of ParUtils::ArrangeValues (include/par/par_utils.h:22-86). */
 gs_status gs_gather(gs_handle* h, int elem_bytes, const void* d_src, const uint64_t* d_idx, void* d_dst,
