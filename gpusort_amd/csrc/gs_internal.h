@@ -96,19 +96,30 @@ uint32_t merge_tile_elems();
 uint32_t merge_plan(const MergeJob* jobs, uint32_t njobs, MergeJobDev* out_jobs, std::vector<uint32_t>* tile_job);
 // Runs every job: run a of a job is read from keys_a/vals_a + a_off, run b
 // from keys_b/vals_b + b_off.  d_split needs ntiles entries of scratch.
+// lex_vxf: when given (and there is a payload) elements compare as (key,
+// payload) pairs, the payload ordered by this transform.
 hipError_t merge_tiles(int key_bytes, int val_bytes, const void* keys_a, const void* keys_b, const void* vals_a,
                        const void* vals_b, void* dst_keys, void* dst_vals, const MergeJobDev* d_jobs,
                        const uint32_t* d_tile_job, uint32_t ntiles, const XformParams& xf, uint64_t* d_split,
-                       hipStream_t stream);
+                       hipStream_t stream, const XformParams* lex_vxf = nullptr);
 
 // ---------------------------------------------------------------- splitting
 // Bucket boundaries of sorted runs.  Run r is keys[run_off[r] .. run_off[r+1])
 // and is cut by splitters[spl_first[r] .. spl_first[r] + nspl).  Output:
 // bounds[r*(nspl+2) + j], j = 0..nspl+1: run start, then run start +
 // lower_bound (upper_bound if `upper`) of each splitter, then run end.
+// With lex_vxf (and a payload of val_bytes), runs and splitters are (key,
+// payload) pairs compared lexicographically, the payload ordered by lex_vxf.
 hipError_t bucket_bounds(int key_bytes, const void* keys, const uint64_t* d_run_off, uint32_t nruns,
                          const void* d_splitters, const uint32_t* d_spl_first, uint32_t nspl, int upper,
-                         const XformParams& xf, uint64_t* d_bounds, hipStream_t stream);
+                         const XformParams& xf, uint64_t* d_bounds, hipStream_t stream, int val_bytes = 0,
+                         const void* vals = nullptr, const void* d_spl_vals = nullptr,
+                         const XformParams* lex_vxf = nullptr);
+// out[i] = src[idx[i]] for n elements of 4 or 8 bytes.
+hipError_t gather_elems(int bytes, const void* src, const uint64_t* d_idx, void* out, uint32_t n, hipStream_t stream);
+// Sets *d_flag (not cleared here) when a[0..n) is not non-decreasing under xf.
+hipError_t unsorted_flag(int bytes, const void* a, uint64_t n, const XformParams& xf, uint32_t* d_flag,
+                         hipStream_t stream);
 
 // ---------------------------------------------------------------- utilities
 hipError_t fill_u32(uint32_t* p, uint64_t n, uint32_t v, hipStream_t stream);

@@ -33,31 +33,52 @@ __device__ inline uint64_t merge_path_search(uint64_t diag, uint64_t na, uint64_
   return lo;
 }
 
-template <typename U>
+// Element compared by a merge: the ordered key, and with LEX also the ordered
+// payload as the second field (the lexicographic order of std::pair under
+// the lifted comparator, include/common.h:156-164).
+template <typename U, typename V, bool LEX>
+struct MKey {
+  U k;
+  V v;
+  __device__ bool operator<=(const MKey& o) const {
+    if constexpr (LEX) return k < o.k || (k == o.k && v <= o.v);
+    return k <= o.k;
+  }
+};
+
+template <typename U, typename V, bool LEX>
 __global__ __launch_bounds__(256) void merge_partition_kernel(const U* __restrict__ keys_a,
                                                               const U* __restrict__ keys_b,
+                                                              const V* __restrict__ vals_a,
+                                                              const V* __restrict__ vals_b,
                                                               const MergeJobDev* __restrict__ jobs,
                                                               const uint32_t* __restrict__ tile_job,
                                                               uint32_t ntiles, uint32_t tile_elems, Xform<U> xf,
-                                                              uint64_t* __restrict__ a_split) {
+                                                              Xform<V> vxf, uint64_t* __restrict__ a_split) {
   const uint32_t t = blockIdx.x * 256 + threadIdx.x;
   if (t >= ntiles) return;
   const MergeJobDev jb = jobs[tile_job[t]];
   const uint64_t diag = (uint64_t)(t - jb.tile0) * tile_elems;
   const U* A = keys_a + jb.a_off;
   const U* B = keys_b + jb.b_off;
-  a_split[t] = merge_path_search<U>(
-      diag, jb.a_len, jb.b_len, [&](uint64_t i) { return to_ordered(A[i], xf); },
-      [&](uint64_t i) { return to_ordered(B[i], xf); });
+  const V* AV = vals_a + jb.a_off;
+  const V* BV = vals_b + jb.b_off;
+  using E = MKey<U, V, LEX>;
+  a_split[t] = merge_path_search<E>(
+      diag, jb.a_len, jb.b_len,
+      [&](uint64_t i) { return E{to_ordered(A[i], xf), LEX ? to_ordered(AV[i], vxf) : V(0)}; },
+      [&](uint64_t i) { return E{to_ordered(B[i], xf), LEX ? to_ordered(BV[i], vxf) : V(0)}; });
 }
 
-template <typename U, typename V, bool KV, int THREADS, int ITEMS>
+template <typename U, typename V, bool KV, bool LEX, int THREADS, int ITEMS>
 __global__ __launch_bounds__(THREADS) void merge_tile_kernel(const U* __restrict__ ska, const U* __restrict__ skb,
                                                              const V* __restrict__ sva, const V* __restrict__ svb,
                                                              U* __restrict__ dk, V* __restrict__ dv,
                                                              const MergeJobDev* __restrict__ jobs,
                                                              const uint32_t* __restrict__ tile_job,
-                                                             const uint64_t* __restrict__ a_split, Xform<U> xf) {
+                                                             const uint64_t* __restrict__ a_split, Xform<U> xf,
+                                                             Xform<V> vxf) {
+  using E = MKey<U, V, LEX>;
   constexpr uint32_t TILE = THREADS * ITEMS;
   __shared__ U s_k[TILE];
   __shared__ V s_v[KV ? TILE : 1];
@@ -89,15 +110,16 @@ __global__ __launch_bounds__(THREADS) void merge_tile_kernel(const U* __restrict
   __syncthreads();
 
   const uint32_t diag = (tid * ITEMS < tot) ? tid * ITEMS : tot;
-  uint32_t ia = (uint32_t)merge_path_search<U>(
-      diag, na, nb, [&](uint64_t i) { return s_k[i]; }, [&](uint64_t i) { return s_k[na + i]; });
+  auto elem = [&](uint32_t i) { return E{s_k[i], LEX ? to_ordered(s_v[i], vxf) : V(0)}; };
+  uint32_t ia = (uint32_t)merge_path_search<E>(
+      diag, na, nb, [&](uint64_t i) { return elem((uint32_t)i); }, [&](uint64_t i) { return elem(na + (uint32_t)i); });
   uint32_t ib = diag - ia;
   U ok[ITEMS];
   uint32_t osrc[ITEMS];
 #pragma unroll
   for (int k = 0; k < ITEMS; ++k) {
     if (diag + k < tot) {
-      const bool take_a = ib >= nb || (ia < na && s_k[ia] <= s_k[na + ib]);
+      const bool take_a = ib >= nb || (ia < na && elem(ia) <= elem(na + ib));
       const uint32_t src = take_a ? ia : na + ib;
       ok[k] = s_k[src];
       osrc[k] = src;
@@ -140,37 +162,49 @@ constexpr int kMergeItems = 8;
 
 uint32_t merge_tile_elems() { return kMergeThreads * kMergeItems; }
 
-template <typename U, int VB>
+template <typename U, int VB, bool LEX>
 static hipError_t merge_launch(const void* ska, const void* skb, const void* sva, const void* svb, void* dk,
-                               void* dv, const MergeJobDev* d_jobs,
-                               const uint32_t* d_tile_job, uint32_t ntiles, const XformParams& xp, uint64_t* d_split,
-                               hipStream_t stream) {
+                               void* dv, const MergeJobDev* d_jobs, const uint32_t* d_tile_job, uint32_t ntiles,
+                               const XformParams& xp, const XformParams& vp, uint64_t* d_split, hipStream_t stream) {
   using V = typename MValT<VB>::T;
   constexpr bool KV = VB != 0;
   const Xform<U> xf{(U)xp.float_mask, (U)xp.sign_flip, (U)xp.desc_mask};
-  hipLaunchKernelGGL(merge_partition_kernel<U>, dim3((ntiles + 255) / 256), dim3(256), 0, stream,
-                     reinterpret_cast<const U*>(ska), reinterpret_cast<const U*>(skb), d_jobs, d_tile_job, ntiles,
-                     merge_tile_elems(), xf, d_split);
-  hipLaunchKernelGGL((merge_tile_kernel<U, V, KV, kMergeThreads, kMergeItems>), dim3(ntiles), dim3(kMergeThreads), 0,
-                     stream, reinterpret_cast<const U*>(ska), reinterpret_cast<const U*>(skb),
+  const Xform<V> vxf{(V)vp.float_mask, (V)vp.sign_flip, (V)vp.desc_mask};
+  hipLaunchKernelGGL((merge_partition_kernel<U, V, LEX>), dim3((ntiles + 255) / 256), dim3(256), 0, stream,
+                     reinterpret_cast<const U*>(ska), reinterpret_cast<const U*>(skb),
+                     reinterpret_cast<const V*>(sva), reinterpret_cast<const V*>(svb), d_jobs, d_tile_job, ntiles,
+                     merge_tile_elems(), xf, vxf, d_split);
+  hipLaunchKernelGGL((merge_tile_kernel<U, V, KV, LEX, kMergeThreads, kMergeItems>), dim3(ntiles),
+                     dim3(kMergeThreads), 0, stream, reinterpret_cast<const U*>(ska), reinterpret_cast<const U*>(skb),
                      reinterpret_cast<const V*>(sva), reinterpret_cast<const V*>(svb), reinterpret_cast<U*>(dk),
-                     reinterpret_cast<V*>(dv), d_jobs, d_tile_job, d_split, xf);
+                     reinterpret_cast<V*>(dv), d_jobs, d_tile_job, d_split, xf, vxf);
   return hipGetLastError();
 }
 
 hipError_t merge_tiles(int kb, int vb, const void* ska, const void* skb, const void* sva, const void* svb, void* dk,
                        void* dv, const MergeJobDev* d_jobs, const uint32_t* d_tile_job, uint32_t ntiles,
-                       const XformParams& xf, uint64_t* d_split, hipStream_t stream) {
+                       const XformParams& xf, uint64_t* d_split, hipStream_t stream, const XformParams* lex_vxf) {
   if (ntiles == 0) return hipSuccess;
+  const XformParams vz{0, 0, 0};
+  const bool lex = lex_vxf != nullptr && vb != 0;
+  const XformParams& vp = lex ? *lex_vxf : vz;
+#define GS_MERGE_CASE(UT, VBV)                                                                                    \
+  return lex ? merge_launch<UT, VBV, true>(ska, skb, sva, svb, dk, dv, d_jobs, d_tile_job, ntiles, xf, vp, d_split, \
+                                           stream)                                                                \
+             : merge_launch<UT, VBV, false>(ska, skb, sva, svb, dk, dv, d_jobs, d_tile_job, ntiles, xf, vp,       \
+                                            d_split, stream)
   if (kb == 4) {
-    if (vb == 0) return merge_launch<uint32_t, 0>(ska, skb, sva, svb, dk, dv, d_jobs, d_tile_job, ntiles, xf, d_split, stream);
-    if (vb == 4) return merge_launch<uint32_t, 4>(ska, skb, sva, svb, dk, dv, d_jobs, d_tile_job, ntiles, xf, d_split, stream);
-    if (vb == 8) return merge_launch<uint32_t, 8>(ska, skb, sva, svb, dk, dv, d_jobs, d_tile_job, ntiles, xf, d_split, stream);
+    if (vb == 0) return merge_launch<uint32_t, 0, false>(ska, skb, sva, svb, dk, dv, d_jobs, d_tile_job, ntiles, xf, vp,
+                                                         d_split, stream);
+    if (vb == 4) GS_MERGE_CASE(uint32_t, 4);
+    if (vb == 8) GS_MERGE_CASE(uint32_t, 8);
   } else if (kb == 8) {
-    if (vb == 0) return merge_launch<uint64_t, 0>(ska, skb, sva, svb, dk, dv, d_jobs, d_tile_job, ntiles, xf, d_split, stream);
-    if (vb == 4) return merge_launch<uint64_t, 4>(ska, skb, sva, svb, dk, dv, d_jobs, d_tile_job, ntiles, xf, d_split, stream);
-    if (vb == 8) return merge_launch<uint64_t, 8>(ska, skb, sva, svb, dk, dv, d_jobs, d_tile_job, ntiles, xf, d_split, stream);
+    if (vb == 0) return merge_launch<uint64_t, 0, false>(ska, skb, sva, svb, dk, dv, d_jobs, d_tile_job, ntiles, xf, vp,
+                                                         d_split, stream);
+    if (vb == 4) GS_MERGE_CASE(uint64_t, 4);
+    if (vb == 8) GS_MERGE_CASE(uint64_t, 8);
   }
+#undef GS_MERGE_CASE
   return hipErrorInvalidValue;
 }
 

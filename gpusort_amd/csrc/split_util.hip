@@ -5,18 +5,22 @@
 //   HykSort / HyperQuickSort: lower_bound  (include/helpers/hyksort_helper.h:563-568,
 //                                           include/helpers/hyperquicksort_helper.h:366-368)
 //   SampleSort:               upper_bound  (include/helpers/samplesort_helper.h:525-537)
-// One lane per (run, splitter); the splitter values are first staged in LDS.
+// One lane per (run, splitter), a binary search over the sorted run.  With a
+// payload the runs are compared as (key, payload) pairs, the order the
+// reference's rounds use (std::pair under the lifted comparator,
+// include/common.h:156-164).
 #include "gs_device.h"
 #include "gs_internal.h"
 
 namespace gs {
 
-template <typename U>
-__global__ __launch_bounds__(256) void bucket_bounds_kernel(const U* __restrict__ keys,
+template <typename U, typename V, bool LEX>
+__global__ __launch_bounds__(256) void bucket_bounds_kernel(const U* __restrict__ keys, const V* __restrict__ vals,
                                                             const uint64_t* __restrict__ run_off, uint32_t nruns,
-                                                            const U* __restrict__ spl,
+                                                            const U* __restrict__ spl, const V* __restrict__ spl_v,
                                                             const uint32_t* __restrict__ spl_first, uint32_t nspl,
-                                                            int upper, Xform<U> xf, uint64_t* __restrict__ bounds) {
+                                                            int upper, Xform<U> xf, Xform<V> vxf,
+                                                            uint64_t* __restrict__ bounds) {
   const uint64_t g = (uint64_t)blockIdx.x * 256 + threadIdx.x;
   const uint64_t per_run = (uint64_t)nspl + 2;
   if (g >= (uint64_t)nruns * per_run) return;
@@ -29,12 +33,24 @@ __global__ __launch_bounds__(256) void bucket_bounds_kernel(const U* __restrict_
   } else if (j == nspl + 1) {
     out = hi0;
   } else {
-    const U s = to_ordered(spl[spl_first[r] + j - 1], xf);
+    const uint64_t si = spl_first[r] + j - 1;
+    const U s = to_ordered(spl[si], xf);
+    const V sv = LEX ? to_ordered(spl_v[si], vxf) : V(0);
     uint64_t lo = lo0, hi = hi0;
     while (lo < hi) {
       const uint64_t mid = (lo + hi) >> 1;
       const U k = to_ordered(keys[mid], xf);
-      const bool go_right = upper ? (k <= s) : (k < s);
+      bool less, equal;
+      if constexpr (LEX) {
+        const V v = to_ordered(vals[mid], vxf);
+        less = k < s || (k == s && v < sv);
+        equal = k == s && v == sv;
+      } else {
+        less = k < s;
+        equal = k == s;
+      }
+      // lower_bound: first element not less than s; upper_bound: first greater.
+      const bool go_right = upper ? (less || equal) : less;
       if (go_right)
         lo = mid + 1;
       else
@@ -45,22 +61,93 @@ __global__ __launch_bounds__(256) void bucket_bounds_kernel(const U* __restrict_
   bounds[g] = out;
 }
 
+template <typename U>
+static hipError_t bounds_u(int vb, const void* keys, const void* vals, const uint64_t* d_run_off, uint32_t nruns,
+                           const void* d_spl, const void* d_spl_v, const uint32_t* d_spl_first, uint32_t nspl,
+                           int upper, const XformParams& xp, const XformParams* vp, uint64_t* d_bounds,
+                           hipStream_t stream) {
+  const uint64_t total = (uint64_t)nruns * (nspl + 2);
+  const uint32_t blocks = (uint32_t)((total + 255) / 256);
+  const Xform<U> xf{(U)xp.float_mask, (U)xp.sign_flip, (U)xp.desc_mask};
+  if (vp == nullptr || vb == 0) {
+    const Xform<uint32_t> vz{0, 0, 0};
+    hipLaunchKernelGGL((bucket_bounds_kernel<U, uint32_t, false>), dim3(blocks), dim3(256), 0, stream,
+                       reinterpret_cast<const U*>(keys), nullptr, d_run_off, nruns, reinterpret_cast<const U*>(d_spl),
+                       nullptr, d_spl_first, nspl, upper, xf, vz, d_bounds);
+  } else if (vb == 4) {
+    const Xform<uint32_t> vxf{(uint32_t)vp->float_mask, (uint32_t)vp->sign_flip, (uint32_t)vp->desc_mask};
+    hipLaunchKernelGGL((bucket_bounds_kernel<U, uint32_t, true>), dim3(blocks), dim3(256), 0, stream,
+                       reinterpret_cast<const U*>(keys), reinterpret_cast<const uint32_t*>(vals), d_run_off, nruns,
+                       reinterpret_cast<const U*>(d_spl), reinterpret_cast<const uint32_t*>(d_spl_v), d_spl_first,
+                       nspl, upper, xf, vxf, d_bounds);
+  } else {
+    const Xform<uint64_t> vxf{vp->float_mask, vp->sign_flip, vp->desc_mask};
+    hipLaunchKernelGGL((bucket_bounds_kernel<U, uint64_t, true>), dim3(blocks), dim3(256), 0, stream,
+                       reinterpret_cast<const U*>(keys), reinterpret_cast<const uint64_t*>(vals), d_run_off, nruns,
+                       reinterpret_cast<const U*>(d_spl), reinterpret_cast<const uint64_t*>(d_spl_v), d_spl_first,
+                       nspl, upper, xf, vxf, d_bounds);
+  }
+  return hipGetLastError();
+}
+
 hipError_t bucket_bounds(int kb, const void* keys, const uint64_t* d_run_off, uint32_t nruns, const void* d_spl,
                          const uint32_t* d_spl_first, uint32_t nspl, int upper, const XformParams& xp,
-                         uint64_t* d_bounds, hipStream_t stream) {
+                         uint64_t* d_bounds, hipStream_t stream, int vb, const void* vals, const void* d_spl_v,
+                         const XformParams* lex_vxf) {
   const uint64_t total = (uint64_t)nruns * (nspl + 2);
   if (total == 0) return hipSuccess;
-  const uint32_t blocks = (uint32_t)((total + 255) / 256);
-  if (kb == 4) {
+  if (kb == 4)
+    return bounds_u<uint32_t>(vb, keys, vals, d_run_off, nruns, d_spl, d_spl_v, d_spl_first, nspl, upper, xp, lex_vxf,
+                              d_bounds, stream);
+  return bounds_u<uint64_t>(vb, keys, vals, d_run_off, nruns, d_spl, d_spl_v, d_spl_first, nspl, upper, xp, lex_vxf,
+                            d_bounds, stream);
+}
+
+// Gathers elements idx[i] of a device array (keys or payloads) into out[i].
+template <typename T>
+__global__ __launch_bounds__(256) void gather_small_kernel(const T* __restrict__ src, const uint64_t* __restrict__ idx,
+                                                           T* __restrict__ out, uint32_t n) {
+  const uint32_t i = blockIdx.x * 256 + threadIdx.x;
+  if (i < n) out[i] = src[idx[i]];
+}
+
+hipError_t gather_elems(int bytes, const void* src, const uint64_t* d_idx, void* out, uint32_t n, hipStream_t stream) {
+  if (n == 0) return hipSuccess;
+  const uint32_t blocks = (n + 255) / 256;
+  if (bytes == 4)
+    hipLaunchKernelGGL(gather_small_kernel<uint32_t>, dim3(blocks), dim3(256), 0, stream,
+                       reinterpret_cast<const uint32_t*>(src), d_idx, reinterpret_cast<uint32_t*>(out), n);
+  else
+    hipLaunchKernelGGL(gather_small_kernel<uint64_t>, dim3(blocks), dim3(256), 0, stream,
+                       reinterpret_cast<const uint64_t*>(src), d_idx, reinterpret_cast<uint64_t*>(out), n);
+  return hipGetLastError();
+}
+
+// Flag set when some element is ordered before its predecessor under the
+// transform (a run that is not non-decreasing in the requested order).
+template <typename T>
+__global__ __launch_bounds__(256) void unsorted_flag_kernel(const T* __restrict__ a, uint64_t n, Xform<T> xf,
+                                                            uint32_t* __restrict__ flag) {
+  const uint64_t stride = (uint64_t)gridDim.x * 256;
+  bool bad = false;
+  for (uint64_t i = (uint64_t)blockIdx.x * 256 + threadIdx.x + 1; i < n; i += stride)
+    bad |= to_ordered(a[i], xf) < to_ordered(a[i - 1], xf);
+  if (__syncthreads_or(bad) && threadIdx.x == 0) atomicOr(flag, 1u);
+}
+
+hipError_t unsorted_flag(int bytes, const void* a, uint64_t n, const XformParams& xp, uint32_t* d_flag,
+                         hipStream_t stream) {
+  if (n < 2) return hipSuccess;
+  uint64_t blocks = (n + 255) / 256;
+  if (blocks > 4096) blocks = 4096;
+  if (bytes == 4) {
     const Xform<uint32_t> xf{(uint32_t)xp.float_mask, (uint32_t)xp.sign_flip, (uint32_t)xp.desc_mask};
-    hipLaunchKernelGGL(bucket_bounds_kernel<uint32_t>, dim3(blocks), dim3(256), 0, stream,
-                       reinterpret_cast<const uint32_t*>(keys), d_run_off, nruns,
-                       reinterpret_cast<const uint32_t*>(d_spl), d_spl_first, nspl, upper, xf, d_bounds);
+    hipLaunchKernelGGL(unsorted_flag_kernel<uint32_t>, dim3((uint32_t)blocks), dim3(256), 0, stream,
+                       reinterpret_cast<const uint32_t*>(a), n, xf, d_flag);
   } else {
     const Xform<uint64_t> xf{xp.float_mask, xp.sign_flip, xp.desc_mask};
-    hipLaunchKernelGGL(bucket_bounds_kernel<uint64_t>, dim3(blocks), dim3(256), 0, stream,
-                       reinterpret_cast<const uint64_t*>(keys), d_run_off, nruns,
-                       reinterpret_cast<const uint64_t*>(d_spl), d_spl_first, nspl, upper, xf, d_bounds);
+    hipLaunchKernelGGL(unsorted_flag_kernel<uint64_t>, dim3((uint32_t)blocks), dim3(256), 0, stream,
+                       reinterpret_cast<const uint64_t*>(a), n, xf, d_flag);
   }
   return hipGetLastError();
 }

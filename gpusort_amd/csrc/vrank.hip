@@ -1,6 +1,45 @@
-// Virtual-rank per-rank pipeline (gs_vrank_sort): placeholder until the round
-// driver lands; reports the call as unsupported.
+// Per-rank pipeline over p virtual ranks held as segments of one device array
+// (gs_vrank_sort).
+//
+// The reference runs one MPI process per rank (SortAmongProcesses:
+// include/helpers/hyksort_helper.h:453-536, samplesort_helper.h:420-501,
+// hyperquicksort_helper.h:261-346).  Here every rank is a device segment, the
+// MPI exchange is addressing (a received bucket is read where its sender
+// holds it), and all ranks of a round are processed by one launch per step:
+//
+//   local sort   every rank's run sorted (radix / LDS tile sort per rank)
+//   samples      regular samples of every rank gathered on the device and
+//                sorted per group (tile sort / radix sort)
+//   splitters    picked from the sorted samples by the rule below
+//   multisplit   bucket bounds of every run by binary search
+//                (lower_bound for HykSort / HyperQuickSort, upper_bound for
+//                SampleSort)
+//   exchange     bucket sizes -> new rank offsets (host scan of p*k counts)
+//   merge        merge-path tree over each destination's received runs,
+//                reading them in place in the senders' runs
+//
+// Order: with a payload, elements compare as (key, payload) pairs under the
+// lifted comparator (std::less/greater of std::pair, include/common.h:
+// 156-164), the order the reference's rounds use.  The local sort is made
+// lexicographic by a stable payload sort first; it is skipped when the
+// payloads are already non-decreasing in that order (e.g. original indexes).
+//
+// Splitter rules (the reference draws HykSort / HyperQuickSort samples with
+// rand(), which no re-run can reproduce; these rules are deterministic and
+// the oracle re-runs them exactly):
+//   HykSort         s regular samples run[(j+1)*n/(s+1)] per rank of a group;
+//                   splitter i = sorted_samples[(i+1)*|S|/k - 1], i < k-1
+//   SampleSort      p-1 samples run[(i+1)*n/p] per rank (samplesort_helper.h:
+//                   402-405); ascending: splitter r = S[(r+1)(p-1)-1],
+//                   descending: S[(r+1)(p-1)] (:411-416)
+//   HyperQuickSort  s regular samples per rank; pivot = the first sample whose
+//                   global lower_bound rank is closest to N_group/2 (:251-256)
 #include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <vector>
 
 #include "../../include/gpusort_hip.h"
 #include "gs_capi_impl.h"
@@ -8,20 +47,603 @@
 
 using namespace gs;
 
+namespace {
+
+struct Pipe {
+  gs_handle* h;
+  gs_dtype kdt, vdt;
+  int kb, vb;
+  bool desc, kv;
+  XformParams kx, vx;
+  char* kbuf[2];
+  char* vbuf[2];
+  int cur;
+  uint64_t n;
+  // host-side aux staging (device copies live in the aux arena slot)
+  std::vector<char> stage;
+};
+
+bool is_pow2(uint64_t x) { return x && !(x & (x - 1)); }
+
+// Sorted-order sort of a device segment: keys-only, or (key, payload)
+// lexicographic when lex_values is set, else stable by key.
+gs_status sort_segment(Pipe& P, char* k, char* v, uint64_t len, bool lex_values) {
+  if (len <= 1) return GS_OK;
+  if (P.kv) {
+    if (lex_values) {
+      gs_status st = sort_device(P.h, P.vdt, v, P.kdt, k, len, P.desc);
+      if (st != GS_OK) return st;
+    }
+    return sort_device(P.h, P.kdt, k, P.vdt, v, len, P.desc);
+  }
+  return sort_device(P.h, P.kdt, k, GS_NONE, nullptr, len, P.desc);
+}
+
+// Device aux area: a list of (host source, bytes) blocks uploaded together.
+struct AuxLayout {
+  std::vector<std::pair<const void*, size_t>> blocks;
+  std::vector<size_t> offs;
+  size_t total = 0;
+  size_t add(const void* src, size_t bytes) {
+    offs.push_back(total);
+    blocks.push_back({src, bytes});
+    const size_t o = total;
+    total = align_up(total + (bytes ? bytes : 8), 256);
+    return o;
+  }
+};
+
+gs_status upload(Pipe& P, AuxLayout& L, char** base) {
+  void* ws = nullptr;
+  hipError_t e = P.h->arena.get(Arena::kPipeAux, L.total, P.h->stream, &ws);
+  if (e != hipSuccess) return from_hip(e, "pipeline workspace");
+  *base = reinterpret_cast<char*>(ws);
+  P.stage.assign(L.total, 0);
+  for (size_t i = 0; i < L.blocks.size(); ++i)
+    if (L.blocks[i].first && L.blocks[i].second) memcpy(P.stage.data() + L.offs[i], L.blocks[i].first, L.blocks[i].second);
+  e = hipMemcpyAsync(ws, P.stage.data(), L.total, hipMemcpyHostToDevice, P.h->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(P.h->stream);  // stage buffer is reused by the next upload
+  return from_hip(e, "pipeline upload");
+}
+
+// Gathers element positions `idx` of the current runs (keys and payloads),
+// sorts consecutive groups (group g = [gofs[g], gofs[g+1]) of the gathered
+// list) in the pipeline order, and returns them on the host.
+gs_status gather_sorted(Pipe& P, const std::vector<uint64_t>& idx, const std::vector<uint64_t>& gofs,
+                        std::vector<char>* hk, std::vector<char>* hv) {
+  const uint32_t m = (uint32_t)idx.size();
+  hk->assign((size_t)m * P.kb, 0);
+  hv->assign(P.kv ? (size_t)m * P.vb : 0, 0);
+  if (m == 0) return GS_OK;
+  AuxLayout L;
+  const size_t o_idx = L.add(idx.data(), m * 8);
+  const size_t o_k = L.add(nullptr, (size_t)m * P.kb);
+  const size_t o_v = L.add(nullptr, P.kv ? (size_t)m * P.vb : 0);
+  char* base = nullptr;
+  gs_status st = upload(P, L, &base);
+  if (st != GS_OK) return st;
+  const uint64_t* d_idx = reinterpret_cast<const uint64_t*>(base + o_idx);
+  hipError_t e = gather_elems(P.kb, P.kbuf[P.cur], d_idx, base + o_k, m, P.h->stream);
+  if (e == hipSuccess && P.kv) e = gather_elems(P.vb, P.vbuf[P.cur], d_idx, base + o_v, m, P.h->stream);
+  if (e != hipSuccess) return from_hip(e, "gather samples");
+  for (size_t g = 0; g + 1 < gofs.size(); ++g) {
+    const uint64_t a = gofs[g], b = gofs[g + 1];
+    st = sort_segment(P, base + o_k + a * P.kb, P.kv ? base + o_v + a * P.vb : nullptr, b - a, P.kv);
+    if (st != GS_OK) return st;
+  }
+  e = hipMemcpyAsync(hk->data(), base + o_k, (size_t)m * P.kb, hipMemcpyDeviceToHost, P.h->stream);
+  if (e == hipSuccess && P.kv)
+    e = hipMemcpyAsync(hv->data(), base + o_v, (size_t)m * P.vb, hipMemcpyDeviceToHost, P.h->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(P.h->stream);
+  return from_hip(e, "read samples");
+}
+
+// Bucket bounds of every run of the current buffer: run r cut by splitters
+// spl[first[r] .. first[r]+nspl).  Returns bounds[r*(nspl+2) + j].
+gs_status run_bounds(Pipe& P, const std::vector<uint64_t>& off, const std::vector<char>& spl_k,
+                     const std::vector<char>& spl_v, const std::vector<uint32_t>& first, uint32_t nspl, bool upper,
+                     std::vector<uint64_t>* bounds) {
+  const uint32_t nruns = (uint32_t)off.size() - 1;
+  bounds->assign((size_t)nruns * (nspl + 2), 0);
+  AuxLayout L;
+  const size_t o_off = L.add(off.data(), off.size() * 8);
+  const size_t o_sk = L.add(spl_k.data(), spl_k.size());
+  const size_t o_sv = L.add(P.kv ? spl_v.data() : nullptr, P.kv ? spl_v.size() : 0);
+  const size_t o_first = L.add(first.data(), first.size() * 4);
+  const size_t o_b = L.add(nullptr, bounds->size() * 8);
+  char* base = nullptr;
+  gs_status st = upload(P, L, &base);
+  if (st != GS_OK) return st;
+  hipError_t e = bucket_bounds(P.kb, P.kbuf[P.cur], reinterpret_cast<const uint64_t*>(base + o_off), nruns, base + o_sk,
+                               reinterpret_cast<const uint32_t*>(base + o_first), nspl, upper ? 1 : 0, P.kx,
+                               reinterpret_cast<uint64_t*>(base + o_b), P.h->stream, P.kv ? P.vb : 0,
+                               P.kv ? P.vbuf[P.cur] : nullptr, P.kv ? base + o_sv : nullptr, P.kv ? &P.vx : nullptr);
+  if (e == hipSuccess)
+    e = hipMemcpyAsync(bounds->data(), base + o_b, bounds->size() * 8, hipMemcpyDeviceToHost, P.h->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(P.h->stream);
+  return from_hip(e, "bucket bounds");
+}
+
+struct Run {
+  uint64_t off, len;
+};
+
+// Merges, for every destination, its list of runs (read from the current
+// buffer, in list order, ties to the earlier run) into one run placed at
+// dest_off[d] of the result.  Pairwise merge-path levels ping-pong between
+// the two buffers; the result buffer becomes current.
+gs_status merge_lists(Pipe& P, std::vector<std::vector<Run>> lists, const std::vector<uint64_t>& dest_off) {
+  for (;;) {
+    bool any = false;
+    for (auto& l : lists) any |= l.size() > 1;
+    bool first_level_pending = false;
+    for (size_t d = 0; d < lists.size(); ++d)
+      if (lists[d].size() == 1 && lists[d][0].off != dest_off[d] && lists[d][0].len) first_level_pending = true;
+    if (!any && !first_level_pending) break;
+    // One level: pairs (0,1), (2,3), ...; a lone last run is copied (merged
+    // with an empty run) so every level writes the whole buffer it owns.
+    std::vector<MergeJob> jobs;
+    std::vector<std::vector<Run>> next(lists.size());
+    for (size_t d = 0; d < lists.size(); ++d) {
+      uint64_t pos = dest_off[d];
+      const auto& l = lists[d];
+      for (size_t j = 0; j < l.size(); j += 2) {
+        const Run a = l[j];
+        const Run b = (j + 1 < l.size()) ? l[j + 1] : Run{a.off, 0};
+        jobs.push_back(MergeJob{a.off, a.len, b.off, b.len, pos});
+        next[d].push_back(Run{pos, a.len + b.len});
+        pos += a.len + b.len;
+      }
+    }
+    std::vector<MergeJobDev> dj(jobs.size());
+    std::vector<uint32_t> tile_job;
+    const uint32_t ntiles = merge_plan(jobs.data(), (uint32_t)jobs.size(), dj.data(), &tile_job);
+    if (ntiles) {
+      AuxLayout L;
+      const size_t o_j = L.add(dj.data(), dj.size() * sizeof(MergeJobDev));
+      const size_t o_t = L.add(tile_job.data(), tile_job.size() * 4);
+      const size_t o_s = L.add(nullptr, (size_t)ntiles * 8);
+      char* base = nullptr;
+      gs_status st = upload(P, L, &base);
+      if (st != GS_OK) return st;
+      const int src = P.cur, dst = P.cur ^ 1;
+      hipError_t e = merge_tiles(P.kb, P.kv ? P.vb : 0, P.kbuf[src], P.kbuf[src], P.kv ? P.vbuf[src] : nullptr,
+                                 P.kv ? P.vbuf[src] : nullptr, P.kbuf[dst], P.kv ? P.vbuf[dst] : nullptr,
+                                 reinterpret_cast<const MergeJobDev*>(base + o_j),
+                                 reinterpret_cast<const uint32_t*>(base + o_t), ntiles, P.kx,
+                                 reinterpret_cast<uint64_t*>(base + o_s), P.h->stream, P.kv ? &P.vx : nullptr);
+      if (e == hipSuccess) e = hipStreamSynchronize(P.h->stream);
+      if (e != hipSuccess) return from_hip(e, "merge level");
+    }
+    P.cur ^= 1;
+    lists.swap(next);
+  }
+  return GS_OK;
+}
+
+// ParUtils::KeepNoProcessHasEmptyData (include/par/par_utils.h:103-175) on
+// the size vector: data keeps its global order, rank boundaries move.  The
+// gathered sizes are not updated inside the loop and the rank after a
+// handled partner is skipped, as in the reference.
+void keep_no_empty(std::vector<uint64_t>& sizes) {
+  const int n_p = (int)sizes.size();
+  const std::vector<uint64_t> nelems = sizes;
+  for (int i = 0; i < n_p; i++) {
+    if (nelems[i] == 0) {
+      int partner = i + 1;
+      int empty_cnt = 1;
+      bool use_prev = false;
+      if (i == n_p - 1) {
+        partner = n_p - 2;
+        use_prev = true;
+      } else {
+        while (partner < n_p && nelems[partner] == 0) {
+          partner++;
+          empty_cnt++;
+        }
+        if (partner == n_p) {
+          partner = i - 1;
+          use_prev = true;
+        }
+      }
+      if (partner < 0) break;
+      const uint64_t size_sync = nelems[partner] / (uint64_t)(empty_cnt + 1);
+      const int last = use_prev ? n_p : partner;
+      for (int j = i; j < last; j++) sizes[j] = size_sync;
+      sizes[partner] -= (uint64_t)empty_cnt * size_sync;
+      if (use_prev) i = n_p;
+      if (i < n_p - 1) i = partner + 1;
+    }
+  }
+}
+
+std::vector<uint64_t> offsets_of(const std::vector<uint64_t>& sizes) {
+  std::vector<uint64_t> off(sizes.size() + 1, 0);
+  for (size_t i = 0; i < sizes.size(); ++i) off[i + 1] = off[i] + sizes[i];
+  return off;
+}
+
+struct Report {
+  gs_vrank_report* r;
+  int kb;
+  void add_splitters(const std::vector<char>& k) {
+    if (!r) return;
+    const uint32_t cnt = (uint32_t)(k.size() / kb);
+    for (uint32_t i = 0; i < cnt; ++i) {
+      if (r->splitters && r->n_splitters < r->max_splitters)
+        memcpy(reinterpret_cast<char*>(r->splitters) + (size_t)r->n_splitters * kb, k.data() + (size_t)i * kb, kb);
+      r->n_splitters++;
+    }
+  }
+  void add_bucket(uint64_t s) {
+    if (!r) return;
+    if (r->bucket_sizes && r->n_bucket_sizes < r->max_bucket_sizes) r->bucket_sizes[r->n_bucket_sizes] = s;
+    r->n_bucket_sizes++;
+  }
+};
+
+// Regular sample positions of run [o, o+len): o + (j+1)*len/(s+1).
+void regular_samples(uint64_t o, uint64_t len, uint32_t s, std::vector<uint64_t>* idx) {
+  if (len == 0) return;
+  for (uint32_t j = 0; j < s; ++j) idx->push_back(o + (uint64_t)(j + 1) * len / (s + 1));
+}
+
+gs_status hyksort_rounds(Pipe& P, std::vector<uint64_t>& sizes, uint32_t k_way, uint32_t s, Report& rep,
+                         double* max_err, uint32_t* rounds) {
+  const uint32_t p = (uint32_t)sizes.size();
+  uint64_t total = 0;
+  for (auto z : sizes) total += z;
+  uint32_t n_p = p;
+  while (n_p > 1) {
+    const uint32_t kk = std::min(k_way, n_p);
+    const uint32_t blk = n_p / kk;
+    const uint32_t groups = p / n_p;
+    std::vector<uint64_t> off = offsets_of(sizes);
+    // samples, group by group
+    std::vector<uint64_t> idx, gofs{0};
+    for (uint32_t g = 0; g < groups; ++g) {
+      for (uint32_t r = g * n_p; r < (g + 1) * n_p; ++r) regular_samples(off[r], sizes[r], s, &idx);
+      gofs.push_back(idx.size());
+    }
+    std::vector<char> sk, sv;
+    gs_status st = gather_sorted(P, idx, gofs, &sk, &sv);
+    if (st != GS_OK) return st;
+    // splitters: group g's kk-1 splitters; empty groups cut nothing
+    std::vector<char> spl_k((size_t)groups * (kk - 1) * P.kb, 0), spl_v((size_t)groups * (kk - 1) * P.vb, 0);
+    std::vector<bool> group_empty(groups, false);
+    for (uint32_t g = 0; g < groups; ++g) {
+      const uint64_t m = gofs[g + 1] - gofs[g];
+      if (m == 0) {
+        group_empty[g] = true;
+        continue;
+      }
+      std::vector<char> gk;
+      for (uint32_t i = 0; i + 1 < kk; ++i) {
+        int64_t at = (int64_t)((uint64_t)(i + 1) * m / kk) - 1;
+        if (at < 0) at = 0;
+        const uint64_t src = gofs[g] + (uint64_t)at;
+        const size_t dsti = (size_t)g * (kk - 1) + i;
+        memcpy(spl_k.data() + dsti * P.kb, sk.data() + src * P.kb, P.kb);
+        if (P.kv) memcpy(spl_v.data() + dsti * P.vb, sv.data() + src * P.vb, P.vb);
+        gk.insert(gk.end(), sk.data() + src * P.kb, sk.data() + (src + 1) * P.kb);
+      }
+      rep.add_splitters(gk);
+    }
+    std::vector<uint32_t> first(p);
+    for (uint32_t r = 0; r < p; ++r) first[r] = (r / n_p) * (kk - 1);
+    std::vector<uint64_t> bnd;
+    st = run_bounds(P, off, spl_k, spl_v, first, kk - 1, false, &bnd);
+    if (st != GS_OK) return st;
+    auto B = [&](uint32_t r, uint32_t i) { return bnd[(size_t)r * (kk + 1) + i]; };  // i = 0..kk
+    for (uint32_t r = 0; r < p; ++r) {
+      const uint32_t g = r / n_p;
+      for (uint32_t i = 0; i < kk; ++i) rep.add_bucket(group_empty[g] ? 0 : B(r, i + 1) - B(r, i));
+    }
+    // splitter quality: |global rank - (i+1) * N_group / k| / N
+    for (uint32_t g = 0; g < groups; ++g) {
+      if (group_empty[g]) continue;
+      uint64_t gtot = 0;
+      for (uint32_t r = g * n_p; r < (g + 1) * n_p; ++r) gtot += sizes[r];
+      for (uint32_t i = 0; i + 1 < kk; ++i) {
+        uint64_t grank = 0;
+        for (uint32_t r = g * n_p; r < (g + 1) * n_p; ++r) grank += B(r, i + 1) - B(r, 0);
+        const double err = std::fabs((double)grank - (double)(i + 1) * (double)gtot / kk) / (double)std::max<uint64_t>(total, 1);
+        *max_err = std::max(*max_err, err);
+      }
+    }
+    // exchange: bucket i of group-local rank lr goes to group-local rank
+    // blk*i + lr % blk (hyksort_helper.h:558, :584); runs merged in source order
+    std::vector<uint64_t> nsizes(p, 0);
+    std::vector<std::vector<Run>> lists(p);
+    for (uint32_t g = 0; g < groups; ++g) {
+      const uint32_t g0 = g * n_p;
+      for (uint32_t dl = 0; dl < n_p; ++dl) {
+        const uint32_t i = dl / blk;
+        for (uint32_t lr = dl % blk; lr < n_p; lr += blk) {
+          const uint32_t r = g0 + lr;
+          const uint64_t a = group_empty[g] ? off[r] : B(r, i);
+          const uint64_t b = group_empty[g] ? (i == 0 && lr == dl ? off[r + 1] : off[r]) : B(r, i + 1);
+          if (b > a) lists[g0 + dl].push_back(Run{a, b - a});
+          nsizes[g0 + dl] += b - a;
+        }
+      }
+    }
+    st = merge_lists(P, lists, offsets_of(nsizes));
+    if (st != GS_OK) return st;
+    sizes = nsizes;
+    n_p = blk;
+    (*rounds)++;
+  }
+  return GS_OK;
+}
+
+gs_status samplesort_round(Pipe& P, std::vector<uint64_t>& sizes, Report& rep, uint32_t* rounds) {
+  const uint32_t p = (uint32_t)sizes.size();
+  if (p == 1) return GS_OK;
+  std::vector<uint64_t> off = offsets_of(sizes);
+  std::vector<uint64_t> idx;
+  for (uint32_t r = 0; r < p; ++r)
+    for (uint32_t i = 0; i + 1 < p; ++i) idx.push_back(off[r] + (uint64_t)(i + 1) * sizes[r] / p);
+  std::vector<uint64_t> gofs{0, idx.size()};
+  std::vector<char> sk, sv;
+  gs_status st = gather_sorted(P, idx, gofs, &sk, &sv);
+  if (st != GS_OK) return st;
+  std::vector<char> spl_k((size_t)(p - 1) * P.kb), spl_v((size_t)(p - 1) * P.vb);
+  for (uint32_t r = 0; r + 1 < p; ++r) {
+    const uint64_t src = P.desc ? (uint64_t)(r + 1) * (p - 1) : (uint64_t)(r + 1) * (p - 1) - 1;
+    memcpy(spl_k.data() + (size_t)r * P.kb, sk.data() + src * P.kb, P.kb);
+    if (P.kv) memcpy(spl_v.data() + (size_t)r * P.vb, sv.data() + src * P.vb, P.vb);
+  }
+  rep.add_splitters(spl_k);
+  std::vector<uint32_t> first(p, 0);
+  std::vector<uint64_t> bnd;
+  st = run_bounds(P, off, spl_k, spl_v, first, p - 1, true, &bnd);
+  if (st != GS_OK) return st;
+  auto B = [&](uint32_t r, uint32_t i) { return bnd[(size_t)r * (p + 1) + i]; };
+  std::vector<uint64_t> nsizes(p, 0);
+  std::vector<std::vector<Run>> lists(p);
+  for (uint32_t r = 0; r < p; ++r)
+    for (uint32_t d = 0; d < p; ++d) {
+      const uint64_t a = B(r, d), b = B(r, d + 1);
+      rep.add_bucket(b - a);
+      if (b > a) lists[d].push_back(Run{a, b - a});
+      nsizes[d] += b - a;
+    }
+  st = merge_lists(P, lists, offsets_of(nsizes));
+  if (st != GS_OK) return st;
+  sizes = nsizes;
+  *rounds = 1;
+  return GS_OK;
+}
+
+gs_status hyperquicksort_rounds(Pipe& P, std::vector<uint64_t>& sizes, uint32_t s, Report& rep, uint32_t* rounds) {
+  const uint32_t p = (uint32_t)sizes.size();
+  uint32_t n_p = p;
+  while (n_p > 1) {
+    const uint32_t groups = p / n_p;
+    std::vector<uint64_t> off = offsets_of(sizes);
+    std::vector<uint64_t> gtot(groups, 0);
+    for (uint32_t r = 0; r < p; ++r) gtot[r / n_p] += sizes[r];
+    // samples in rank order (unsorted: the pivot rule scans them in gather order)
+    std::vector<uint64_t> idx, gofs{0};
+    for (uint32_t g = 0; g < groups; ++g) {
+      if (gtot[g])
+        for (uint32_t r = g * n_p; r < (g + 1) * n_p; ++r) regular_samples(off[r], sizes[r], s, &idx);
+      gofs.push_back(idx.size());
+    }
+    std::vector<char> sk, sv;
+    {
+      std::vector<uint64_t> one{0, 0};  // gather only, no sort
+      gs_status st = gather_sorted(P, idx, one, &sk, &sv);
+      if (st != GS_OK) return st;
+    }
+    // global lower_bound rank of every sample within its group
+    std::vector<uint32_t> first(p);
+    uint32_t maxm = 0;
+    for (uint32_t g = 0; g < groups; ++g) maxm = std::max<uint32_t>(maxm, (uint32_t)(gofs[g + 1] - gofs[g]));
+    std::vector<char> spl_k, spl_v;
+    std::vector<uint64_t> pivots(groups, 0);
+    std::vector<bool> active(groups, false);
+    if (maxm) {
+      // pad every group's sample list to maxm so one bounds launch serves all
+      spl_k.assign((size_t)groups * maxm * P.kb, 0);
+      spl_v.assign((size_t)groups * maxm * P.vb, 0);
+      for (uint32_t g = 0; g < groups; ++g) {
+        const uint64_t m = gofs[g + 1] - gofs[g];
+        for (uint64_t x = 0; x < maxm; ++x) {
+          const uint64_t srcx = gofs[g] + (m ? std::min<uint64_t>(x, m - 1) : 0);
+          if (m == 0) continue;
+          memcpy(spl_k.data() + ((size_t)g * maxm + x) * P.kb, sk.data() + srcx * P.kb, P.kb);
+          if (P.kv) memcpy(spl_v.data() + ((size_t)g * maxm + x) * P.vb, sv.data() + srcx * P.vb, P.vb);
+        }
+      }
+      for (uint32_t r = 0; r < p; ++r) first[r] = (r / n_p) * maxm;
+      std::vector<uint64_t> bnd;
+      gs_status st = run_bounds(P, off, spl_k, spl_v, first, maxm, false, &bnd);
+      if (st != GS_OK) return st;
+      for (uint32_t g = 0; g < groups; ++g) {
+        const uint64_t m = gofs[g + 1] - gofs[g];
+        if (m == 0 || gtot[g] == 0) continue;
+        active[g] = true;
+        const int64_t half = (int64_t)(gtot[g] / 2);
+        uint64_t best = 0;
+        int64_t best_rank = -1;
+        for (uint64_t x = 0; x < m; ++x) {
+          int64_t gr = 0;
+          for (uint32_t r = g * n_p; r < (g + 1) * n_p; ++r)
+            gr += (int64_t)(bnd[(size_t)r * (maxm + 2) + x + 1] - bnd[(size_t)r * (maxm + 2)]);
+          if (x == 0 || std::llabs(gr - half) < std::llabs(best_rank - half)) {
+            best = x;
+            best_rank = gr;
+          }
+        }
+        pivots[g] = gofs[g] + best;
+      }
+    }
+    // pivot bounds per run
+    std::vector<char> pk((size_t)groups * P.kb, 0), pv((size_t)groups * P.vb, 0);
+    std::vector<char> rep_k;
+    for (uint32_t g = 0; g < groups; ++g) {
+      if (!active[g]) continue;
+      memcpy(pk.data() + (size_t)g * P.kb, sk.data() + pivots[g] * P.kb, P.kb);
+      if (P.kv) memcpy(pv.data() + (size_t)g * P.vb, sv.data() + pivots[g] * P.vb, P.vb);
+      rep_k.insert(rep_k.end(), pk.data() + (size_t)g * P.kb, pk.data() + (size_t)(g + 1) * P.kb);
+    }
+    rep.add_splitters(rep_k);
+    for (uint32_t r = 0; r < p; ++r) first[r] = r / n_p;
+    std::vector<uint64_t> pb;
+    gs_status st = run_bounds(P, off, pk, pv, first, 1, false, &pb);
+    if (st != GS_OK) return st;
+    std::vector<uint64_t> split(p);
+    for (uint32_t r = 0; r < p; ++r) {
+      split[r] = active[r / n_p] ? pb[(size_t)r * 3 + 1] : off[r + 1];
+      rep.add_bucket(split[r] - off[r]);
+      rep.add_bucket(off[r + 1] - split[r]);
+    }
+    // low half keeps [start, split), high half keeps [split, end); partner =
+    // lr + cmp_p0 - new_p0 (hyperquicksort_helper.h:358-362); kept run first
+    const uint32_t split_id = (n_p - 1) / 2;
+    std::vector<uint64_t> nsizes(p, 0);
+    std::vector<std::vector<Run>> lists(p);
+    for (uint32_t g = 0; g < groups; ++g) {
+      const uint32_t g0 = g * n_p;
+      for (uint32_t lr = 0; lr < n_p; ++lr) {
+        const uint32_t r = g0 + lr;
+        if (!active[g]) {
+          if (sizes[r]) lists[r].push_back(Run{off[r], sizes[r]});
+          nsizes[r] = sizes[r];
+          continue;
+        }
+        const bool low = lr <= split_id;
+        const uint32_t new_p0 = low ? 0 : split_id + 1;
+        const uint32_t cmp_p0 = low ? split_id + 1 : 0;
+        uint32_t partner = lr + cmp_p0 - new_p0;
+        if (partner >= n_p) partner = n_p - 1;
+        const uint32_t q = g0 + partner;
+        const Run kept = low ? Run{off[r], split[r] - off[r]} : Run{split[r], off[r + 1] - split[r]};
+        const Run recv = low ? Run{off[q], split[q] - off[q]} : Run{split[q], off[q + 1] - split[q]};
+        if (kept.len) lists[r].push_back(kept);
+        if (recv.len) lists[r].push_back(recv);
+        nsizes[r] = kept.len + recv.len;
+      }
+    }
+    st = merge_lists(P, lists, offsets_of(nsizes));
+    if (st != GS_OK) return st;
+    sizes = nsizes;
+    n_p /= 2;
+    (*rounds)++;
+  }
+  return GS_OK;
+}
+
+}  // namespace
+
 extern "C" gs_status gs_vrank_sort(gs_handle* h, gs_algorithm algo, gs_dtype kdt, void* d_keys, gs_dtype vdt,
                                    void* d_vals, const uint64_t* h_rank_off, uint32_t p, uint32_t k_way,
                                    uint32_t samples_per_rank, int descending, gs_vrank_report* report) {
-  (void)h;
-  (void)algo;
-  (void)kdt;
-  (void)d_keys;
-  (void)vdt;
-  (void)d_vals;
-  (void)h_rank_off;
-  (void)p;
-  (void)k_way;
-  (void)samples_per_rank;
-  (void)descending;
-  (void)report;
-  return fail(GS_UNSUPPORTED, "gs_vrank_sort is not built yet");
+  if (!h) return fail(GS_INVALID_ARG, "null handle");
+  if (!dtype_ok(kdt)) return fail(GS_INVALID_ARG, "unsupported key type %d", (int)kdt);
+  if (vdt != GS_NONE && !dtype_ok(vdt)) return fail(GS_INVALID_ARG, "unsupported value type %d", (int)vdt);
+  if (p == 0 || !h_rank_off) return fail(GS_INVALID_ARG, "need p >= 1 rank offsets");
+  if (algo != GS_HYKSORT && algo != GS_SAMPLESORT && algo != GS_HYPERQUICKSORT)
+    return fail(GS_INVALID_ARG, "unknown algorithm %d", (int)algo);
+  if (algo == GS_HYKSORT && k_way < 2) return fail(GS_INVALID_ARG, "k_way must be at least 2");
+  if ((algo == GS_HYKSORT || algo == GS_HYPERQUICKSORT) && samples_per_rank == 0)
+    return fail(GS_INVALID_ARG, "samples_per_rank must be at least 1");
+  std::vector<uint64_t> sizes(p);
+  for (uint32_t r = 0; r < p; ++r) {
+    if (h_rank_off[r + 1] < h_rank_off[r]) return fail(GS_INVALID_ARG, "rank offsets decrease at rank %u", r);
+    sizes[r] = h_rank_off[r + 1] - h_rank_off[r];
+    // Every rank must hold data (hyksort_helper.h:105, samplesort_helper.h:69).
+    if (sizes[r] == 0) return fail(GS_EMPTY_INPUT, "No data to sort.");
+  }
+  if (h_rank_off[0] != 0) return fail(GS_INVALID_ARG, "rank offsets must start at 0");
+  const uint64_t n = h_rank_off[p];
+  if (!d_keys || (vdt != GS_NONE && !d_vals)) return fail(GS_INVALID_ARG, "null pointer");
+  // Power-of-two checks fall back to SampleSort, as HYKSORT_CHECK and
+  // HYPERQUICKSORT_CHECK do (hyksort_common_vector.inl:31-43,
+  // hyperquicksort_common_vector.inl:30-41).
+  if (algo == GS_HYKSORT && (!is_pow2(p) || !is_pow2(k_way))) algo = GS_SAMPLESORT;
+  if (algo == GS_HYPERQUICKSORT && !is_pow2(p)) algo = GS_SAMPLESORT;
+  if (algo == GS_SAMPLESORT && p > 1 && n < 5ull * p * p)
+    return fail(GS_UNSUPPORTED, "SampleSort with N < 5p^2 takes the distributed bitonic path, not built");
+
+  DeviceScope ds(h->device);
+  Pipe P;
+  P.h = h;
+  P.kdt = kdt;
+  P.vdt = vdt;
+  P.kb = dtype_bytes(kdt);
+  P.vb = val_bytes_of(vdt);
+  P.kv = vdt != GS_NONE;
+  P.desc = descending != 0;
+  P.kx = make_xform((int)kdt, P.desc);
+  P.vx = P.kv ? make_xform((int)vdt, P.desc) : XformParams{0, 0, 0};
+  P.n = n;
+  P.cur = 0;
+  P.kbuf[0] = reinterpret_cast<char*>(d_keys);
+  P.vbuf[0] = reinterpret_cast<char*>(d_vals);
+  void *k1 = nullptr, *v1 = nullptr;
+  hipError_t e = h->arena.get(Arena::kPipeKeys, n * (size_t)P.kb, h->stream, &k1);
+  if (e == hipSuccess && P.kv) e = h->arena.get(Arena::kPipeVals, n * (size_t)P.vb, h->stream, &v1);
+  if (e != hipSuccess) return fail(GS_OUT_OF_MEMORY, "Couldn't allocate device vector");
+  P.kbuf[1] = reinterpret_cast<char*>(k1);
+  P.vbuf[1] = reinterpret_cast<char*>(v1);
+
+  if (report) {
+    report->rounds = 0;
+    report->n_splitters = 0;
+    report->n_bucket_sizes = 0;
+    report->max_rank_error = 0.0;
+  }
+  Report rep{report, P.kb};
+
+  // Local sort of every rank; lexicographic with a payload unless the
+  // payloads are already non-decreasing in the requested order.
+  bool lex_values = false;
+  if (P.kv) {
+    void* ws = nullptr;
+    e = h->arena.get(Arena::kPipeAux, 256, h->stream, &ws);
+    if (e != hipSuccess) return from_hip(e, "pipeline workspace");
+    uint32_t flag = 0;
+    e = hipMemcpyAsync(ws, &flag, 4, hipMemcpyHostToDevice, h->stream);
+    if (e == hipSuccess) e = unsorted_flag(P.vb, d_vals, n, P.vx, reinterpret_cast<uint32_t*>(ws), h->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(&flag, ws, 4, hipMemcpyDeviceToHost, h->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
+    if (e != hipSuccess) return from_hip(e, "payload order check");
+    lex_values = flag != 0;
+  }
+  for (uint32_t r = 0; r < p; ++r) {
+    gs_status st = sort_segment(P, P.kbuf[0] + h_rank_off[r] * P.kb, P.kv ? P.vbuf[0] + h_rank_off[r] * P.vb : nullptr,
+                                sizes[r], lex_values);
+    if (st != GS_OK) return st;
+  }
+
+  uint32_t rounds = 0;
+  double max_err = 0.0;
+  gs_status st = GS_OK;
+  if (p > 1) {
+    if (algo == GS_HYKSORT)
+      st = hyksort_rounds(P, sizes, k_way, samples_per_rank, rep, &max_err, &rounds);
+    else if (algo == GS_SAMPLESORT)
+      st = samplesort_round(P, sizes, rep, &rounds);
+    else
+      st = hyperquicksort_rounds(P, sizes, samples_per_rank, rep, &rounds);
+    if (st != GS_OK) return st;
+    keep_no_empty(sizes);
+  }
+  if (P.cur != 0) {
+    e = copy_bytes(P.kbuf[0], P.kbuf[1], n * (size_t)P.kb, h->stream);
+    if (e == hipSuccess && P.kv) e = copy_bytes(P.vbuf[0], P.vbuf[1], n * (size_t)P.vb, h->stream);
+    if (e != hipSuccess) return from_hip(e, "copy result");
+  }
+  e = hipStreamSynchronize(h->stream);
+  if (e != hipSuccess) return from_hip(e, "pipeline");
+  if (report) {
+    report->rounds = rounds;
+    report->max_rank_error = max_err;
+    if (report->final_rank_sizes)
+      for (uint32_t r = 0; r < p; ++r) report->final_rank_sizes[r] = sizes[r];
+  }
+  return GS_OK;
 }
