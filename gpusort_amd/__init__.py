@@ -135,6 +135,17 @@ _SIGS = {
         [ctypes.c_void_p, ctypes.POINTER(KernelStat), ctypes.c_uint32, ctypes.POINTER(ctypes.c_uint32)],
         ctypes.c_int,
     ),
+    "gs_vrank_sort_host": (
+        [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p,
+         ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_int, ctypes.POINTER(VrankReport)],
+        ctypes.c_int,
+    ),
+    "gs_gather_host": (
+        [ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p, ctypes.c_uint64, ctypes.c_void_p, ctypes.c_void_p,
+         ctypes.c_uint64],
+        ctypes.c_int,
+    ),
+    "gs_rebalance_sizes": ([ctypes.POINTER(ctypes.c_uint64), ctypes.c_uint32], ctypes.c_int),
     "gs_gather": (
         [ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_uint64],
         ctypes.c_int,
@@ -375,6 +386,13 @@ class Handle:
                 int(descending), None if pairs_out is None else pairs_out.ctypes.data, pair_stride, val_offset,
             )
         )
+
+
+def rebalance_sizes(sizes) -> np.ndarray:
+    """Final rank sizes after the empty-rank rebalance (gs_rebalance_sizes)."""
+    s = np.ascontiguousarray(sizes, dtype=np.uint64).copy()
+    _check(load_library().gs_rebalance_sizes(s.ctypes.data_as(ctypes.POINTER(ctypes.c_uint64)), s.size))
+    return s
 
 
 def version() -> str:

@@ -647,3 +647,60 @@ extern "C" gs_status gs_vrank_sort(gs_handle* h, gs_algorithm algo, gs_dtype kdt
   }
   return GS_OK;
 }
+
+extern "C" gs_status gs_vrank_sort_host(gs_handle* h, gs_algorithm algo, gs_dtype kdt, void* h_keys, gs_dtype vdt,
+                                        void* h_vals, const uint64_t* h_rank_off, uint32_t p, uint32_t k_way,
+                                        uint32_t samples_per_rank, int descending, gs_vrank_report* report) {
+  if (!h) return fail(GS_INVALID_ARG, "null handle");
+  if (!dtype_ok(kdt) || (vdt != GS_NONE && !dtype_ok(vdt))) return fail(GS_INVALID_ARG, "unsupported type");
+  if (p == 0 || !h_rank_off) return fail(GS_INVALID_ARG, "need p >= 1 rank offsets");
+  const uint64_t n = h_rank_off[p];
+  if (n > 0 && (!h_keys || (vdt != GS_NONE && !h_vals))) return fail(GS_INVALID_ARG, "null pointer");
+  DeviceScope ds(h->device);
+  const size_t kbytes = n * (size_t)dtype_bytes(kdt), vbytes = n * (size_t)val_bytes_of(vdt);
+  void *dk = nullptr, *dv = nullptr;
+  if (h->arena.get(Arena::kStageKeys, kbytes, h->stream, &dk) != hipSuccess ||
+      (vbytes && h->arena.get(Arena::kStageVals, vbytes, h->stream, &dv) != hipSuccess))
+    return fail(GS_OUT_OF_MEMORY, "Couldn't allocate device vector");
+  hipError_t e = hipMemcpyAsync(dk, h_keys, kbytes, hipMemcpyHostToDevice, h->stream);
+  if (e == hipSuccess && vbytes) e = hipMemcpyAsync(dv, h_vals, vbytes, hipMemcpyHostToDevice, h->stream);
+  if (e != hipSuccess) return from_hip(e, "copy to device");
+  gs_status st = gs_vrank_sort(h, algo, kdt, dk, vdt, dv, h_rank_off, p, k_way, samples_per_rank, descending, report);
+  if (st != GS_OK) return st;
+  e = hipMemcpyAsync(h_keys, dk, kbytes, hipMemcpyDeviceToHost, h->stream);
+  if (e == hipSuccess && vbytes) e = hipMemcpyAsync(h_vals, dv, vbytes, hipMemcpyDeviceToHost, h->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
+  return from_hip(e, "copy to host");
+}
+
+extern "C" gs_status gs_gather_host(gs_handle* h, int elem_bytes, const void* h_src, uint64_t n_src,
+                                    const uint64_t* h_idx, void* h_dst, uint64_t n) {
+  if (!h) return fail(GS_INVALID_ARG, "null handle");
+  if (elem_bytes != 1 && elem_bytes != 2 && elem_bytes != 4 && elem_bytes != 8)
+    return fail(GS_INVALID_ARG, "element size %d not in {1,2,4,8}", elem_bytes);
+  if (n == 0) return GS_OK;
+  if (!h_src || !h_idx || !h_dst) return fail(GS_INVALID_ARG, "null pointer");
+  for (uint64_t i = 0; i < n; ++i)
+    if (h_idx[i] >= n_src) return fail(GS_INVALID_ARG, "index %llu out of range", (unsigned long long)h_idx[i]);
+  DeviceScope ds(h->device);
+  void *ds_src = nullptr, *ds_idx = nullptr, *ds_dst = nullptr;
+  if (h->arena.get(Arena::kStageKeys, n_src * (size_t)elem_bytes, h->stream, &ds_src) != hipSuccess ||
+      h->arena.get(Arena::kStagePairs, n * 8, h->stream, &ds_idx) != hipSuccess ||
+      h->arena.get(Arena::kStageVals, n * (size_t)elem_bytes, h->stream, &ds_dst) != hipSuccess)
+    return fail(GS_OUT_OF_MEMORY, "Couldn't allocate device vector");
+  hipError_t e = hipMemcpyAsync(ds_src, h_src, n_src * (size_t)elem_bytes, hipMemcpyHostToDevice, h->stream);
+  if (e == hipSuccess) e = hipMemcpyAsync(ds_idx, h_idx, n * 8, hipMemcpyHostToDevice, h->stream);
+  if (e == hipSuccess)
+    e = gather_values(elem_bytes, ds_src, reinterpret_cast<const uint64_t*>(ds_idx), ds_dst, n, h->stream);
+  if (e == hipSuccess) e = hipMemcpyAsync(h_dst, ds_dst, n * (size_t)elem_bytes, hipMemcpyDeviceToHost, h->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
+  return from_hip(e, "gather");
+}
+
+extern "C" gs_status gs_rebalance_sizes(uint64_t* sizes, uint32_t p) {
+  if (p && !sizes) return fail(GS_INVALID_ARG, "null sizes");
+  std::vector<uint64_t> s(sizes, sizes + p);
+  keep_no_empty(s);
+  for (uint32_t i = 0; i < p; ++i) sizes[i] = s[i];
+  return GS_OK;
+}

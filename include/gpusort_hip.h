@@ -189,10 +189,25 @@ typedef struct gs_kernel_stat {
 gs_status gs_profile_begin(gs_handle* h);
 gs_status gs_profile_end(gs_handle* h, gs_kernel_stat* out, uint32_t max_out, uint32_t* n_out);
 
+/* Host-memory form of gs_vrank_sort: the arrays are staged through the
+ * handle's arena and copied back in place. */
+gs_status gs_vrank_sort_host(gs_handle* h, gs_algorithm algo, gs_dtype kdt, void* h_keys, gs_dtype vdt, void* h_vals,
+                             const uint64_t* h_rank_off, uint32_t p, uint32_t k_way, uint32_t samples_per_rank,
+                             int descending, gs_vrank_report* report);
+
 /* dst[i] = src[idx[i]] for a payload of elem_bytes (1, 2, 4, 8) bytes.
  * Device form of ParUtils::ArrangeValues (include/par/par_utils.h:22-86). */
 gs_status gs_gather(gs_handle* h, int elem_bytes, const void* d_src, const uint64_t* d_idx, void* d_dst,
                     uint64_t n);
+
+/* Host-memory form of gs_gather: src has n_src elements, dst and idx n. */
+gs_status gs_gather_host(gs_handle* h, int elem_bytes, const void* h_src, uint64_t n_src, const uint64_t* h_idx,
+                         void* h_dst, uint64_t n);
+
+/* Final rank sizes after ParUtils::KeepNoProcessHasEmptyData
+ * (include/par/par_utils.h:103-175): empty ranks take data from their
+ * nearest non-empty neighbour; the data keeps its global order.  Host only. */
+gs_status gs_rebalance_sizes(uint64_t* sizes, uint32_t p);
 
 #ifdef __cplusplus
 }

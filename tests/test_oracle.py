@@ -1,0 +1,253 @@
+"""Pins the CPU oracle (oracle/oracle.cpp) before it is trusted as the checker.
+
+* Golden vectors of the reference's own OmpUtils (tests/golden/, produced by
+  make_golden.py from include/omp/omp_utils.h compiled out of the reference
+  tree): the restatement must reproduce them bit for bit.
+* The reference's compiled OmpUtils itself (oracle/_ref), when built here.
+* Closed forms the reference's tests use as known answers
+  (src/test/arithmetic_progression_checker.h:49-106, ln_checker.h:45-87,
+  log_checker.h:45-87, zero_checker.h:34-74) and library routines (sort,
+  stable argsort, lexsort, searchsorted, cumsum).
+* Brute force over every permutation of tiny inputs, with thread counts that
+  take the split-and-merge path of MergeSort.
+"""
+from __future__ import annotations
+
+import itertools
+import math
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+import oracle
+from gpusort_amd import datagen
+
+GOLDEN = Path(__file__).resolve().parent / "golden" / "omp_utils_golden.npz"
+
+
+@pytest.fixture(autouse=True)
+def restore_threads():
+    yield
+    oracle.set_threads(0)
+
+
+def _golden():
+    return np.load(GOLDEN, allow_pickle=False)
+
+
+@pytest.mark.parametrize("threads", [1, 3, 8])
+def test_golden_merge_sort(threads):
+    g = _golden()
+    oracle.set_threads(threads)
+    tags = sorted({k[: -len("_in")] for k in g.files if k.startswith(f"sort_t{threads}_") and k.endswith("_in")})
+    assert tags
+    for tag in tags:
+        desc = tag.endswith("_d1")
+        got = oracle.merge_sort(g[tag + "_in"], descending=desc)
+        want = g[tag + "_out"]
+        assert got.dtype == want.dtype and np.array_equal(got.view(np.uint8), want.view(np.uint8)), tag
+
+
+@pytest.mark.parametrize("threads", [1, 3, 8])
+def test_golden_pairs_merge(threads):
+    g = _golden()
+    oracle.set_threads(threads)
+    for tag in sorted({k[: -len("_kin")] for k in g.files if k.startswith(f"pairs_t{threads}_") and k.endswith("_kin")}):
+        k, v = oracle.merge_sort_pairs(g[tag + "_kin"], g[tag + "_vin"], descending=tag.endswith("_d1"))
+        assert np.array_equal(k, g[tag + "_kout"]) and np.array_equal(v, g[tag + "_vout"]), tag
+    for tag in sorted({k[: -len("_out")] for k in g.files if k.startswith(f"merge_t{threads}_") and k.endswith("_out")}):
+        assert np.array_equal(oracle.merge(g[tag + "_a"], g[tag + "_b"]), g[tag + "_out"]), tag
+    for tag in sorted({k[: -len("_out")] for k in g.files if k.startswith(f"scan_t{threads}_") and k.endswith("_out")}):
+        assert np.array_equal(oracle.scan(g[tag + "_a"], first=3), g[tag + "_out"]), tag
+
+
+def test_reference_lib_agrees_when_built():
+    if oracle.ref_lib() is None:
+        pytest.skip("oracle/_ref not built (reference tree absent)")
+    for threads in (2, 5, 8):
+        oracle.set_threads(threads)
+        for dtype in (np.uint32, np.float64, np.int64):
+            k = datagen.uniform_u64(threads, 50_003).view(np.int64).astype(dtype) if dtype != np.uint32 else \
+                datagen.uniform_u32(threads, 50_003)
+            assert np.array_equal(oracle.merge_sort(k), oracle.ref_merge_sort(k))
+            assert np.array_equal(oracle.merge_sort(k, True), oracle.ref_merge_sort(k, True))
+        k = datagen.uniform_u32(9, 40_000) % np.uint32(300)
+        v = datagen.uniform_u32(10, 40_000) % np.uint32(7)
+        for desc in (False, True):
+            a = oracle.merge_sort_pairs(k, v, desc)
+            b = oracle.ref_merge_sort_pairs(k, v, desc)
+            assert np.array_equal(a[0], b[0]) and np.array_equal(a[1], b[1])
+
+
+@pytest.mark.parametrize("threads", [1, 4, 8])
+@pytest.mark.parametrize("dtype", [np.uint32, np.int32, np.float32, np.uint64, np.int64, np.float64],
+                         ids=lambda d: np.dtype(d).name)
+def test_merge_sort_equals_sort(threads, dtype):
+    oracle.set_threads(threads)
+    r = datagen.draws(threads * 7 + np.dtype(dtype).itemsize, 33_333)
+    k = (r >> np.uint64(16)).astype(np.float64).astype(dtype) if np.dtype(dtype).kind == "f" else r.view(np.int64).astype(dtype)
+    assert np.array_equal(oracle.merge_sort(k), np.sort(k, kind="stable"))
+    assert np.array_equal(oracle.merge_sort(k, descending=True), np.sort(k)[::-1])
+
+
+def test_arithmetic_progression_known_answer():
+    # ArithmeticProgressionChecker: the sorted output is exactly a1 + i*d,
+    # descending when d < 0 (src/test/arithmetic_progression_checker.h:49-106).
+    oracle.set_threads(6)
+    n = 100_000
+    for a1, d in ((-5000, 3), (7, -2)):
+        ap = (a1 + np.arange(n, dtype=np.int64) * d)
+        shuffled = ap[np.argsort(datagen.uniform_u64(31, n), kind="stable")]
+        assert np.array_equal(oracle.merge_sort(shuffled, descending=d < 0), ap)
+
+
+def test_log_sequences_known_answer():
+    # LnChecker / LogChecker: values log(i) / log10(i) of the global index
+    # (src/test/ln_checker.h:45-87, log_checker.h:45-87).
+    n = 50_000
+    idx = np.arange(1, n + 1, dtype=np.float64)
+    for f in (np.log, np.log10):
+        seq = f(idx)
+        shuffled = seq[np.argsort(datagen.uniform_u64(5, n), kind="stable")]
+        got = oracle.merge_sort(shuffled)
+        assert np.array_equal(got, seq)
+        assert got[0] == 0.0 and got[-1] == f(float(n))
+
+
+def test_all_equal_known_answer():
+    # ZeroChecker: every element equals the one constant (zero_checker.h:34-74).
+    k = np.full(77_777, 42, dtype=np.int32)
+    assert np.array_equal(oracle.merge_sort(k), k)
+    kk, vv = oracle.stable_sort_pairs(k, np.arange(k.size, dtype=np.uint32))
+    assert np.array_equal(vv, np.arange(k.size, dtype=np.uint32))
+
+
+@pytest.mark.parametrize("threads", [2, 3])
+def test_brute_force_permutations(threads):
+    # n >= 2p takes the split + merge path; every permutation of a multiset
+    # with ties must come out sorted, and pairs in lexicographic order.
+    oracle.set_threads(threads)
+    base = np.array([3, 1, 3, 2, 1, 0, 3], dtype=np.int32)
+    for perm in itertools.permutations(range(base.size)):
+        k = base[list(perm)]
+        assert np.array_equal(oracle.merge_sort(k), np.sort(base))
+    vals = np.arange(7, dtype=np.uint32)
+    for perm in itertools.islice(itertools.permutations(range(7)), 0, 5040, 7):
+        k = base[list(perm)]
+        v = vals[list(perm)]
+        gk, gv = oracle.merge_sort_pairs(k, v)
+        order = np.lexsort((v, k))
+        assert np.array_equal(gk, k[order]) and np.array_equal(gv, v[order])
+        sk, sv = oracle.stable_sort_pairs(k, v)
+        st = np.argsort(k, kind="stable")
+        assert np.array_equal(sv, v[st])
+
+
+def test_merge_ties_take_first_run():
+    # std::merge: equal elements of the first range precede those of the second.
+    a = np.array([1, 2, 2, 5], dtype=np.int64)
+    b = np.array([2, 2, 3], dtype=np.int64)
+    assert np.array_equal(oracle.merge(a, b), np.array([1, 2, 2, 2, 2, 3, 5]))
+    for threads in (1, 2, 4):
+        oracle.set_threads(threads)
+        x = np.sort(datagen.uniform_u32(3, 20_000) % np.uint32(500))
+        y = np.sort(datagen.uniform_u32(4, 31_000) % np.uint32(500))
+        assert np.array_equal(oracle.merge(x, y), np.sort(np.concatenate([x, y])))
+
+
+def test_scan_closed_form():
+    for cnt in (1, 50, 1000, 123_457):
+        oracle.set_threads(7)
+        a = (datagen.draws(cnt, cnt) % np.uint64(1000)).astype(np.int64)
+        want = np.concatenate([[11], 11 + np.cumsum(a)[:-1]]) if cnt else a
+        assert np.array_equal(oracle.scan(a, first=11), want)
+
+
+def test_bounds_match_searchsorted():
+    run = np.sort(datagen.uniform_u32(2, 10_000) % np.uint32(2000))
+    spl = np.array([0, 1, 999, 1000, 1999, 2000, 4000], dtype=np.uint32)
+    assert np.array_equal(oracle.bounds(run, spl), np.searchsorted(run, spl, side="left"))
+    assert np.array_equal(oracle.bounds(run, spl, upper=True), np.searchsorted(run, spl, side="right"))
+    rdesc = run[::-1].copy()
+    neg = -rdesc.astype(np.int64)
+    assert np.array_equal(oracle.bounds(rdesc, spl, descending=True),
+                          np.searchsorted(neg, -spl.astype(np.int64), side="left"))
+
+
+def test_keep_no_empty_rules():
+    # Hand-run of ParUtils::KeepNoProcessHasEmptyData (par_utils.h:103-175):
+    # nelems is the gathered vector (not updated in the loop) and the rank
+    # after a handled partner is skipped.
+    assert oracle.keep_no_empty([4, 5, 6]).tolist() == [4, 5, 6]
+    assert oracle.keep_no_empty([0, 10, 7]).tolist() == [5, 5, 7]
+    assert oracle.keep_no_empty([0, 0, 9, 1]).tolist() == [3, 3, 3, 1]
+    assert oracle.keep_no_empty([4, 5, 0]).tolist() == [4, 3, 2]
+    assert oracle.keep_no_empty([5, 0, 0, 9]).tolist() == [5, 3, 3, 3]
+    assert oracle.keep_no_empty([0, 10, 0, 10]).tolist() == [5, 5, 0, 10]
+    assert oracle.keep_no_empty([8, 0, 0]).tolist() == [4, 2, 2]
+    s = datagen.uniform_u32(1, 64) % np.uint32(3)
+    out = oracle.keep_no_empty(s.astype(np.int64))
+    assert out.sum() == s.sum()
+
+
+@pytest.mark.parametrize("algorithm", ["hyksort", "samplesort", "hyperquicksort"])
+def test_vrank_rule_invariants(algorithm):
+    p = 16
+    sizes = 500 + (datagen.draws(4, p) % np.uint64(2000)).astype(np.int64)
+    off = np.concatenate([[0], np.cumsum(sizes)])
+    n = int(off[-1])
+    k = datagen.uniform_u64(8, n) % np.uint64(n // 3)
+    v = np.arange(n, dtype=np.uint32)
+    gk, gv, rep = oracle.vrank(k, off, v, algorithm=algorithm, k_way=4, samples_per_rank=16)
+    order = np.lexsort((v, k))
+    assert np.array_equal(gk, k[order]) and np.array_equal(gv, v[order])
+    assert rep["final_rank_sizes"].sum() == n
+    b = rep["bucket_sizes"]
+    per_round = b.size // max(rep["rounds"], 1)
+    for r in range(rep["rounds"]):
+        assert b[r * per_round:(r + 1) * per_round].sum() == n
+    if algorithm == "hyksort":
+        assert rep["rounds"] == 2 and rep["splitters"].size == 3 + 4 * 3
+        assert 0 <= rep["max_rank_error"] < 0.25
+    # final ranks are contiguous pieces of the global order
+    fo = np.concatenate([[0], np.cumsum(rep["final_rank_sizes"])])
+    assert fo[-1] == n
+
+
+def test_vrank_tiny_hand_case():
+    # p=2, k=2, one sample per rank: rank 0 = [5 1 9 3] -> sorted [1 3 5 9],
+    # sample run[(0+1)*4/2] = 5; rank 1 = [8 2 6 4] -> [2 4 6 8], sample 6.
+    # Sorted samples [5 6]; splitter = S[1*2/2 - 1] = 5.  lower_bound(5):
+    # rank 0 -> [1 3 | 5 9], rank 1 -> [2 4 | 6 8].  Rank 0 receives the low
+    # buckets, rank 1 the high ones.
+    k = np.array([5, 1, 9, 3, 8, 2, 6, 4], dtype=np.uint32)
+    gk, _, rep = oracle.vrank(k, [0, 4, 8], algorithm="hyksort", k_way=2, samples_per_rank=1)
+    assert gk.tolist() == [1, 2, 3, 4, 5, 6, 8, 9]
+    assert rep["splitters"].tolist() == [5]
+    assert rep["bucket_sizes"].tolist() == [2, 2, 2, 2]
+    assert rep["final_rank_sizes"].tolist() == [4, 4]
+    assert rep["max_rank_error"] == pytest.approx(0.0)
+
+
+def test_datagen_streams_are_counter_based():
+    a = datagen.draws(5, 1000)
+    b = datagen.draws(5, 400, start=600)
+    assert np.array_equal(a[600:], b)
+    # splitmix64 of state seed + G: the first output for seed 0 is the
+    # published first value of the splitmix64 sequence seeded with 0.
+    assert int(datagen.draws(0, 1)[0]) == 0xE220A8397B1DCDAF
+    u = datagen.uniform_u32(1, 1 << 16)
+    assert abs(float(u.mean()) / 2**32 - 0.5) < 0.01
+    s = datagen.sorted_u32(3, 100_000)
+    assert np.all(s[1:] >= s[:-1])
+    z = datagen.zipf_u32(2, 200_000)
+    top = np.unique(z, return_counts=True)[1].max() / z.size
+    h = sum(1.0 / k**1.1 for k in range(1, 1 << 20))
+    assert abs(top - 1.0 / h) < 0.01
+    f = datagen.normal_f32(4, 200_000)
+    assert np.all(np.isfinite(f)) and not np.any(np.signbit(f) & (f == 0))
+    assert abs(float(f.std()) / 1e3 - 1.0) < 0.01
+    L = datagen.lognormal_lengths(1, 100_000)
+    assert L.min() >= 1 and L.max() <= 1 << 22
+    assert abs(math.log(np.median(L)) - math.log(2048)) < 0.05
