@@ -18,6 +18,7 @@ from __future__ import annotations
 import argparse
 import json
 import os
+import re
 import sys
 import time
 
@@ -145,7 +146,10 @@ def roofline_of(prof: dict, name: str = "onesweep_pass"):
 
 def cpu_baseline(n_sample: int, reps: int = 3):
     """The reference's OmpUtils::MergeSort on (key, index) pairs (the CPU local
-    sort of HykSort/SampleSort), on the first n_sample keys of the same input."""
+    sort of HykSort/SampleSort) on the host cores, over the first n_sample
+    keys of the same input (all of it by default).  Checked in O(n): keys
+    non-decreasing, payloads a permutation mapping input to output, ties in
+    index order."""
     import oracle
     from gpusort_amd import datagen
 
@@ -156,23 +160,55 @@ def cpu_baseline(n_sample: int, reps: int = 3):
     times = []
     out = None
     for _ in range(reps):
+        out = None
         t0 = time.perf_counter()
         out = fn(keys, vals)
         times.append(time.perf_counter() - t0)
-    perm = np.argsort(keys, kind="stable")
-    ok = bool(np.array_equal(out[1], perm.astype(np.uint32)))
+    ok_k, ok_v = out
+    idx = ok_v.astype(np.int64)
+    ok = bool(np.all(ok_k[1:] >= ok_k[:-1])) and bool(np.all(np.bincount(idx, minlength=n_sample) == 1))
+    ok = ok and bool(np.array_equal(keys[idx], ok_k))
+    ok = ok and bool(np.all((idx[1:] > idx[:-1]) | (ok_k[1:] != ok_k[:-1])))
     t = float(np.median(times))
+    what = "the whole input" if n_sample == (1 << 28) else f"the first {n_sample} keys of the input"
     return {
         "value": round(n_sample / t / 1e9, 4),
         "unit": "Gkeys/s",
         "cores": oracle.threads(),
         "kind": kind,
-        "sample": f"first {n_sample} keys of the uniform u32 input with index payloads, "
-                  f"OmpUtils::MergeSort on std::pair, median of {reps}",
+        "sample": f"{what} (uniform u32 keys, u32 index payloads), OmpUtils::MergeSort on std::pair, "
+                  f"median of {reps}",
         "seconds": round(t, 3),
         "verified": ok,
         "cpu_model": _cpu_model(),
     }
+
+
+def traffic_from_profile(n: int):
+    """HBM bytes per onesweep launch from the committed FETCH_SIZE / WRITE_SIZE
+    summary (rocprofv3 --pmc, scaled by the plain-copy calibration), when it
+    was taken on this workload size."""
+    path = os.path.join(ROOT, "profiles", "pmc_onesweep_kv_u32_2p28.txt")
+    try:
+        text = open(path).read()
+    except OSError:
+        return None
+    m = re.search(r"^traffic_bytes_per_launch\s*=\s*(\d+)", text, re.M)
+    nk = re.search(r"^n_keys\s*=\s*(\d+)", text, re.M)
+    if not m or not nk or int(nk.group(1)) != n:
+        return None
+    return int(m.group(1))
+
+
+def copy_calibration(torch, n: int, steps: int) -> None:
+    """A plain device copy of n uint32 (4n bytes read, 4n written), for
+    scaling the FETCH_SIZE / WRITE_SIZE counters."""
+    src = torch.randint(0, 1 << 30, (n,), dtype=torch.int32, device="cuda")
+    dst = torch.empty_like(src)
+    for _ in range(steps):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    log(f"[bench] copy calibration: {steps} copies of {4 * n} bytes")
 
 
 def _cpu_model() -> str:
@@ -194,7 +230,9 @@ def main() -> int:
     ap.add_argument("--n", type=int, default=1 << 28, help="keys per sort")
     ap.add_argument("--no-extra", action="store_true", help="skip the other distributions")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
-    ap.add_argument("--cpu-n", type=int, default=1 << 25, help="CPU baseline sample size")
+    ap.add_argument("--cpu-n", type=int, default=1 << 28, help="CPU baseline sample size")
+    ap.add_argument("--copy-calib", action="store_true",
+                    help="only run plain copies of --n uint32 (counter calibration) and exit")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -209,12 +247,17 @@ def main() -> int:
     if not torch.cuda.is_available():
         print("no HIP device visible", file=sys.stderr)
         return 1
-    h = g.Handle(0)
     n = args.n
+    if args.copy_calib:
+        copy_calibration(torch, n, args.steps)
+        return 0
+    h = g.Handle(0)
 
     log(f"[bench] headline: KV u32+u32 uniform n={n} steps={args.steps} warmup={args.warmup}")
     head = run_workload(h, g, torch, "uniform", n, args.steps, args.warmup, payload=True, profile=True)
     roof = roofline_of(head["profile"])
+    if roof is not None:
+        roof["traffic"] = traffic_from_profile(n)
     hist = head["profile"].get("radix_hist")
 
     extra = {}

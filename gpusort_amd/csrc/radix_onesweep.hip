@@ -387,8 +387,10 @@ __global__ __launch_bounds__(256) void radix_final_copy_kernel(U* __restrict__ k
 
 template <typename U, int VB>
 struct Geometry;
-// Tile sizes: 512 threads (8 waves); LDS holds one tile of the wider of key
-// and payload, reused for the payload pass.
+// Tile sizes: 32-bit keys with no or 32-bit payloads use 1024 threads x 16
+// keys (16K-key tiles: fewer look-back steps per key), the rest 512 x 12.
+// LDS holds one tile of the wider of key and payload, reused for the payload
+// scatter.
 template <>
 struct Geometry<uint32_t, 0> {
   static constexpr int THREADS = 1024, ITEMS = 16;

@@ -243,10 +243,20 @@ void CaseGlobalIndexesAndArrange() {
 }
 
 void CaseKeepNoProcessHasEmptyData() {
+  // Known answers of the reference's rule (include/par/par_utils.h:103-175):
+  // rank 0 takes half of rank 1; the loop then skips past rank 2 and rank 3
+  // (last) takes nelems[2] / 2 = 0 of the sizes gathered before the call.
   std::vector<uint32_t> k(10, 7u);
-  MpiComm comm = MpiComm::Virtual({0, 10, 0, 0});
-  gpusort::ParUtils::KeepNoProcessHasEmptyData(comm, &k);
-  Report("keep_no_process_has_empty_data", Partitioned(comm, 10));
+  MpiComm a = MpiComm::Virtual({0, 10, 0, 0});
+  gpusort::ParUtils::KeepNoProcessHasEmptyData(a, &k);
+  MpiComm b = MpiComm::Virtual({10, 0, 0, 0});
+  gpusort::ParUtils::KeepNoProcessHasEmptyData(b, &k);
+  MpiComm c = MpiComm::Virtual({3, 0, 4, 3});
+  gpusort::ParUtils::KeepNoProcessHasEmptyData(c, &k);
+  const bool ok = a.Sizes() == std::vector<uint64_t>({5, 5, 0, 0}) &&
+                  b.Sizes() == std::vector<uint64_t>({4, 2, 2, 2}) &&
+                  c.Sizes() == std::vector<uint64_t>({3, 2, 2, 3});
+  Report("keep_no_process_has_empty_data", ok);
 }
 
 template <class Exc, class F>
