@@ -123,6 +123,101 @@ def run_workload(h, g, torch, dist: str, n: int, steps: int, warmup: int, payloa
     return {"gkeys_s": n / (ms * 1e-3) / 1e9, "ms_per_sort": ms, "verified": ok, "profile": prof}
 
 
+def run_batched(h, g, torch, n_arrays: int, steps: int, seed: int = 4):
+    """Many independent arrays: one gs_sort_segments call for the whole batch
+    against one gs_sort_pairs call per array over the same device buffers.
+    Lengths clip(round(lognormal(ln 2048, 2.0)), 1, 2^22), uniform u32 keys,
+    u32 payload = index within the array."""
+    from gpusort_amd import datagen
+
+    lengths = datagen.lognormal_lengths(seed, n_arrays, median=2048, sigma=2.0, lo=1, hi=1 << 22)
+    off = np.concatenate([[0], np.cumsum(lengths)]).astype(np.uint64)
+    n = int(off[-1])
+    keys0 = torch.empty(n, dtype=torch.uint32, device="cuda")
+    h.generate(keys0, g.GEN_UNIFORM_U32, seed)
+    starts = torch.from_numpy(off[:-1].astype(np.int64)).cuda()
+    lens_t = torch.from_numpy(lengths.astype(np.int64)).cuda()
+    vals0 = (torch.arange(n, dtype=torch.int64, device="cuda") - torch.repeat_interleave(starts, lens_t, output_size=n))
+    vals0 = vals0.to(torch.int32).view(torch.uint32)
+    del starts, lens_t
+    d_off = torch.from_numpy(off.view(np.int64)).cuda()
+    torch.cuda.synchronize()
+
+    k, v = keys0.clone(), vals0.clone()
+    h.sort_segments(k, d_off, v)  # warm-up: grows the workspace
+    del k, v
+    copies = [(keys0.clone(), vals0.clone()) for _ in range(steps)]
+    torch.cuda.synchronize()
+    h.profile_begin()
+    t0 = time.perf_counter()
+    for k, v in copies:
+        h.sort_segments(k, d_off, v)
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    prof = h.profile_end()
+    batched_ms = (t1 - t0) * 1e3 / steps
+    bk, bv = copies[-1]
+    del copies[:-1]
+
+    # One call per array, same buffers (the calls are asynchronous: the host
+    # enqueues while the device sorts).
+    ok_k, ok_v = keys0.clone(), vals0.clone()
+    lib = g.load_library()
+    kp, vp, hh = ok_k.data_ptr(), ok_v.data_ptr(), h.h
+    starts_l = [int(x) for x in off[:-1]]
+    lens_l = [int(x) for x in lengths]
+    fn = lib.gs_sort_pairs
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for a, m in zip(starts_l, lens_l):
+        st = fn(hh, g.U32, kp + 4 * a, g.U32, vp + 4 * a, m, 0)
+        if st != 0:
+            raise RuntimeError(f"gs_sort_pairs failed: {lib.gs_last_error()}")
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    single_ms = (t1 - t0) * 1e3
+
+    same = bool(torch.equal(bk.view(torch.int32), ok_k.view(torch.int32))) and \
+        bool(torch.equal(bv.view(torch.int32), ok_v.view(torch.int32)))
+    # Spot check against numpy on the host: the largest array and 32 others.
+    hk = None
+    picks = [int(np.argmax(lengths))] + [int(x) for x in np.linspace(0, n_arrays - 1, 32).astype(np.int64)]
+    spot = True
+    for s_ in picks:
+        a, b = int(off[s_]), int(off[s_ + 1])
+        k_in = keys0[a:b].cpu().numpy()
+        order = np.argsort(k_in, kind="stable")
+        hk = bk[a:b].cpu().numpy()
+        spot = spot and np.array_equal(hk, k_in[order]) and np.array_equal(bv[a:b].cpu().numpy(),
+                                                                             order.astype(np.uint32))
+    del copies, bk, bv, ok_k, ok_v, keys0, vals0
+    torch.cuda.empty_cache()
+
+    classes = {}
+    for lo, hi, name in [(0, 1024, "<=1024"), (1025, 65536, "1025-65536"), (65537, 1 << 40, ">65536")]:
+        m = (lengths >= lo) & (lengths <= hi)
+        classes[name] = {"arrays": int(m.sum()), "keys_frac": round(float(lengths[m].sum()) / n, 4)}
+    seg = prof.get("seg_onesweep_pass")
+    pass_stats = None
+    if seg and seg["launches"]:
+        avg = seg["total_ms"] / seg["launches"]
+        gbs = seg["bytes"] / seg["launches"] / (avg * 1e-3) / 1e9
+        pass_stats = {"avg_ms": round(avg, 4), "GB/s": round(gbs, 1), "frac": round(gbs / HBM_PEAK_GBS, 4),
+                      "launches": seg["launches"]}
+    return {
+        "arrays": n_arrays,
+        "keys": n,
+        "gkeys_s": round(n / (batched_ms * 1e-3) / 1e9, 3),
+        "ms": round(batched_ms, 3),
+        "one_call_per_array_ms": round(single_ms, 3),
+        "one_call_per_array_gkeys_s": round(n / (single_ms * 1e-3) / 1e9, 3),
+        "speedup_vs_one_call": round(single_ms / batched_ms, 2),
+        "verified": same and spot,
+        "classes": classes,
+        "seg_onesweep_pass": pass_stats,
+    }
+
+
 def roofline_of(prof: dict, name: str = "onesweep_pass"):
     st = prof.get(name)
     if not st or st["launches"] == 0:
@@ -230,6 +325,7 @@ def main() -> int:
     ap.add_argument("--n", type=int, default=1 << 28, help="keys per sort")
     ap.add_argument("--no-extra", action="store_true", help="skip the other distributions")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument("--arrays", type=int, default=100000, help="arrays in the batched workload")
     ap.add_argument("--cpu-n", type=int, default=1 << 28, help="CPU baseline sample size")
     ap.add_argument("--copy-calib", action="store_true",
                     help="only run plain copies of --n uint32 (counter calibration) and exit")
@@ -271,6 +367,8 @@ def main() -> int:
             r = run_workload(h, g, torch, dist, n, 3, 1, payload=True, profile=False)
             extra[f"kv_u32_{dist}"] = {"gkeys_s": round(r["gkeys_s"], 3), "ms": round(r["ms_per_sort"], 3),
                                        "verified": r["verified"]}
+        log(f"[bench] batched: {args.arrays} arrays, one call vs one call per array")
+        extra["batched_arrays_kv_u32"] = run_batched(h, g, torch, args.arrays, 3)
         log("[bench] KV f32 normal 2^27")
         r = run_workload(h, g, torch, "normal_f32", n // 2, 3, 1, payload=True, profile=False)
         extra["kv_f32_normal_2^27"] = {"gkeys_s": round(r["gkeys_s"], 3), "ms": round(r["ms_per_sort"], 3),

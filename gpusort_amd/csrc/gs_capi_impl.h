@@ -41,7 +41,7 @@ class Profiler : public LaunchTimer {
 class Arena {
  public:
   enum Slot { kRadix = 0, kStageKeys, kStageVals, kStagePairs, kMerge, kSmall, kPipeKeys, kPipeVals, kPipeAux,
-              kSegAux, kSlots };
+              kSegAux, kSegWork, kSlots };
   Arena() {
     for (int i = 0; i < kSlots; ++i) {
       ptr_[i] = nullptr;

@@ -61,6 +61,30 @@ hipError_t radix_sort(int key_bytes, int val_bytes, void* keys, void* vals, uint
 // Tile geometry of the onesweep pass for a key/value width pair.
 uint32_t radix_tile_keys(int key_bytes, int val_bytes);
 
+// Segmented onesweep: many segments of one array sorted by one launch per
+// digit.  Segment slot s is [seg_range[2s], seg_range[2s+1]) of the array,
+// shorter than 2^30 keys, cut into tiles of radix_tile_keys() keys numbered
+// segment after segment (seg_first[s] = its first tile, tile_seg[t] = the
+// slot of tile t); slots [0, n_multi) are the ones with more than one tile.
+// All maps are device arrays.  n_total bounds every element offset (the
+// ping-pong buffer in the workspace mirrors the whole array); n_keys is the
+// number of keys in the segments.
+struct SegTiles {
+  const uint32_t* tile_seg;
+  const uint32_t* seg_first;
+  const uint64_t* seg_range;
+  uint64_t tiles;
+  uint32_t nslots;
+  uint32_t n_multi;
+  uint64_t n_total;
+  uint64_t n_keys;
+};
+size_t radix_segments_workspace_bytes(int key_bytes, int val_bytes, uint64_t n_total, uint64_t tiles,
+                                      uint32_t nslots);
+hipError_t radix_sort_segments(int key_bytes, int val_bytes, void* keys, void* vals, const SegTiles& tiles,
+                               const XformParams& xf, void* workspace, size_t workspace_bytes, hipStream_t stream,
+                               LaunchTimer* timer = nullptr);
+
 // ---------------------------------------------------------------- block sort
 // Single-workgroup stable sort of n <= block_sort_max(key_bytes, val_bytes)
 // keys held in LDS (bitonic network over (key, original index) pairs).

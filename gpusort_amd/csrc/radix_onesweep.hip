@@ -153,6 +153,13 @@ struct OnesweepArgs {
   S* status;                   // tiles*256 status words of this pass, zeroed
   SortCtrl* ctrl;
   uint32_t tiles;
+  // Segmented mode only: tiles never straddle segments and the look-back
+  // stops at the segment's first tile.
+  const uint32_t* tile_seg;   // tile -> segment slot
+  const uint32_t* seg_first;  // slot -> its first tile
+  const uint64_t* seg_range;  // slot -> [start, end) element offsets, 2 words
+  const uint32_t* seg_base;   // [slot][pass][256] digit starts within the segment
+  int passes;                 // stride of seg_base
 };
 
 // Digit of a key: the ordered image's byte at `shift`.  IDENT is the
@@ -163,10 +170,10 @@ __device__ inline uint32_t key_digit(U key, const Xform<U>& xf, int shift) {
   return digit_of(to_ordered(key, xf), shift);
 }
 
-template <typename U, typename V, bool KV, int THREADS, int ITEMS, typename S, bool IDENT>
+// SEG: segmented mode (every pass runs; pass p reads buffer p & 1).  Idx is
+// the type of element positions: 32 bits whenever they fit.
+template <typename U, typename V, bool KV, int THREADS, int ITEMS, typename S, bool IDENT, bool SEG, typename Idx>
 __global__ __launch_bounds__(THREADS) void onesweep_kernel(OnesweepArgs<U, V, S> a) {
-  // Positions fit 32 bits whenever the 32-bit status word is used (n < 2^30).
-  using Idx = typename std::conditional<sizeof(S) == 4, uint32_t, uint64_t>::type;
   constexpr int WAVES = THREADS / kWave;
   constexpr uint32_t TILE = THREADS * ITEMS;
   constexpr int FLAG_SHIFT = sizeof(S) * 8 - 2;
@@ -186,8 +193,13 @@ __global__ __launch_bounds__(THREADS) void onesweep_kernel(OnesweepArgs<U, V, S>
   __shared__ uint32_t s_tile;
   __shared__ __attribute__((aligned(16))) unsigned char s_buf[TILE * BUF_ELEM];
 
-  const uint32_t src = a.ctrl->src[a.pass];
-  if (src > 1u) return;  // every key has the same digit here: the pass is the identity
+  uint32_t src;
+  if constexpr (SEG) {
+    src = (uint32_t)a.pass & 1u;
+  } else {
+    src = a.ctrl->src[a.pass];
+    if (src > 1u) return;  // every key has the same digit here: the pass is the identity
+  }
   const U* __restrict__ kin = src ? a.keys1 : a.keys0;
   U* __restrict__ kout = src ? a.keys0 : a.keys1;
   const V* __restrict__ vin = src ? a.vals1 : a.vals0;
@@ -205,8 +217,22 @@ __global__ __launch_bounds__(THREADS) void onesweep_kernel(OnesweepArgs<U, V, S>
   }
   lds_barrier();
   const uint32_t tile = s_tile;
-  const uint64_t tile_base = (uint64_t)tile * TILE;
-  const uint64_t remain = a.n - tile_base;
+  uint64_t tile_base, remain;
+  uint32_t first = 0;       // first tile of this tile's segment
+  uint64_t seg_start = 0;   // element offset of the segment
+  const uint64_t* gbase64 = a.digit_base;
+  const uint32_t* gbase32 = nullptr;
+  if constexpr (SEG) {
+    const uint32_t slot = a.tile_seg[tile];
+    first = a.seg_first[slot];
+    seg_start = a.seg_range[2 * slot];
+    tile_base = seg_start + (uint64_t)(tile - first) * TILE;
+    remain = a.seg_range[2 * slot + 1] - tile_base;
+    gbase32 = a.seg_base + ((uint64_t)slot * a.passes + a.pass) * kRadix;
+  } else {
+    tile_base = (uint64_t)tile * TILE;
+    remain = a.n - tile_base;
+  }
   const uint32_t valid = remain < TILE ? (uint32_t)remain : TILE;
   const bool full = valid == TILE;
 
@@ -273,7 +299,9 @@ __global__ __launch_bounds__(THREADS) void onesweep_kernel(OnesweepArgs<U, V, S>
 #pragma unroll
     for (int w = 0; w < WAVES; ++w) total += s_wave_hist[w * kRadix + tid];
     pub = total - ((tid == kRadix - 1) ? (TILE - valid) : 0u);
-    const S word = (tile == 0) ? (FLAG_PRE | (S)(a.digit_base[tid] + pub)) : (FLAG_AGG | (S)pub);
+    uint64_t dbase = 0;
+    if (tile == first) dbase = SEG ? (uint64_t)gbase32[tid] : gbase64[tid];
+    const S word = (tile == first) ? (FLAG_PRE | (S)(dbase + pub)) : (FLAG_AGG | (S)pub);
     status_store(&a.status[(uint64_t)tile * kRadix + tid], word);
     incl = wave_incl_scan(total);
     if (lane == 63) s_wsum[wave] = incl;
@@ -309,8 +337,8 @@ __global__ __launch_bounds__(THREADS) void onesweep_kernel(OnesweepArgs<U, V, S>
   if (tid < kRadix) {
     constexpr int W = kLookback;
     uint64_t excl;
-    if (tile == 0) {
-      excl = a.digit_base[tid];
+    if (tile == first) {
+      excl = SEG ? (uint64_t)gbase32[tid] : gbase64[tid];
     } else {
       excl = 0;
       int64_t t = (int64_t)tile - 1;  // newest predecessor not yet consumed
@@ -319,7 +347,7 @@ __global__ __launch_bounds__(THREADS) void onesweep_kernel(OnesweepArgs<U, V, S>
         S w[W];
 #pragma unroll
         for (int j = 0; j < W; ++j)
-          w[j] = (t - j >= 0) ? status_load_cached(&a.status[(uint64_t)(t - j) * kRadix + tid]) : S(0);
+          w[j] = (t - j >= (int64_t)first) ? status_load_cached(&a.status[(uint64_t)(t - j) * kRadix + tid]) : S(0);
         // A 0 may be a stale cached copy: the newest unconsumed word is
         // re-read device-coherently, so an unpublished word is always polled
         // from memory.
@@ -339,7 +367,7 @@ __global__ __launch_bounds__(THREADS) void onesweep_kernel(OnesweepArgs<U, V, S>
       }
       status_store(&a.status[(uint64_t)tile * kRadix + tid], (S)(FLAG_PRE | (S)(excl + pub)));
     }
-    s_gofs[tid] = (Idx)(excl - local_start);
+    s_gofs[tid] = (Idx)(seg_start + excl - local_start);
   }
   lds_barrier();
 
@@ -481,6 +509,8 @@ template <typename U, int VB, typename S>
 static hipError_t radix_sort_t(U* keys, void* vals_v, uint64_t n, const XformParams& xp, unsigned char* ws,
                                const RadixLayout& L, hipStream_t stream, LaunchTimer* timer) {
   using V = typename ValT<VB>::T;
+  // Positions fit 32 bits whenever the 32-bit status word is used (n < 2^30).
+  using Idx = typename std::conditional<sizeof(S) == 4, uint32_t, uint64_t>::type;
   constexpr bool KV = VB != 0;
   constexpr int THREADS = Geometry<U, VB>::THREADS;
   constexpr int ITEMS = Geometry<U, VB>::ITEMS;
@@ -523,10 +553,10 @@ static hipError_t radix_sort_t(U* keys, void* vals_v, uint64_t n, const XformPar
     a.tiles = (uint32_t)L.tiles;
     if (timer) timer->begin("onesweep_pass", 2 * n * (sizeof(U) + (KV ? sizeof(V) : 0)), stream);
     if (ident)
-      hipLaunchKernelGGL((onesweep_kernel<U, V, KV, THREADS, ITEMS, S, true>), dim3((uint32_t)L.tiles),
+      hipLaunchKernelGGL((onesweep_kernel<U, V, KV, THREADS, ITEMS, S, true, false, Idx>), dim3((uint32_t)L.tiles),
                          dim3(THREADS), 0, stream, a);
     else
-      hipLaunchKernelGGL((onesweep_kernel<U, V, KV, THREADS, ITEMS, S, false>), dim3((uint32_t)L.tiles),
+      hipLaunchKernelGGL((onesweep_kernel<U, V, KV, THREADS, ITEMS, S, false, false, Idx>), dim3((uint32_t)L.tiles),
                          dim3(THREADS), 0, stream, a);
     if (timer) timer->end(stream);
   }
@@ -560,6 +590,199 @@ hipError_t radix_sort(int kb, int vb, void* keys, void* vals, uint64_t n, const 
     if (vb == 0) return radix_sort_s<uint64_t, 0>(k, vals, n, xf, ws, L, stream, timer);
     if (vb == 4) return radix_sort_s<uint64_t, 4>(k, vals, n, xf, ws, L, stream, timer);
     if (vb == 8) return radix_sort_s<uint64_t, 8>(k, vals, n, xf, ws, L, stream, timer);
+  }
+  return hipErrorInvalidValue;
+}
+
+// ------------------------------------------------------------ segmented sort
+//
+// Many segments sorted by one launch sequence: fill, seg_hist, seg_scan, then
+// one onesweep launch per digit for all of them.  Each segment is cut into
+// its own tiles (the last one partial), tiles are numbered segment after
+// segment, and a tile's look-back ends at its segment's first tile, whose
+// prefix is the segment's digit start.  Segments must be shorter than 2^30
+// keys (32-bit look-back words).
+
+template <typename U>
+struct SegHistArgs {
+  const U* keys;
+  Xform<U> xf;
+  int passes;
+  uint32_t tile_keys;
+  const uint32_t* tile_seg;
+  const uint32_t* seg_first;
+  const uint64_t* seg_range;
+  uint32_t n_multi;  // slots [0, n_multi) have several tiles
+  uint32_t* seg_cnt;  // [slot][pass][256]
+  uint32_t* status;   // [pass][tile][256]
+  uint64_t status_per_pass;
+};
+
+// Digit counts of one tile for every pass, added into its segment's counts
+// (atomically for multi-tile segments, whose counts start at zero; stored
+// for single-tile ones); also zeroes the tile's look-back words.
+template <typename U>
+__global__ __launch_bounds__(256) void seg_hist_kernel(SegHistArgs<U> a) {
+  __shared__ uint32_t s_hist[kMaxPasses * kRadix];
+  const uint32_t tid = threadIdx.x, tile = blockIdx.x;
+  const int passes = a.passes;
+  for (uint32_t i = tid; i < (uint32_t)passes * kRadix; i += 256) s_hist[i] = 0;
+  for (int p = 0; p < passes; ++p) a.status[p * a.status_per_pass + (uint64_t)tile * kRadix + tid] = 0;
+  const uint32_t slot = a.tile_seg[tile];
+  const uint64_t start = a.seg_range[2 * slot] + (uint64_t)(tile - a.seg_first[slot]) * a.tile_keys;
+  const uint64_t seg_end = a.seg_range[2 * slot + 1];
+  const uint64_t end = (start + a.tile_keys < seg_end) ? start + a.tile_keys : seg_end;
+  __syncthreads();
+  for (uint64_t i = start + tid; i < end; i += 256) {
+    const U u = to_ordered(a.keys[i], a.xf);
+    for (int p = 0; p < passes; ++p) hist_add<U>(s_hist + p * kRadix, digit_of(u, p * 8));
+  }
+  __syncthreads();
+  uint32_t* dst = a.seg_cnt + (uint64_t)slot * passes * kRadix;
+  if (slot < a.n_multi) {
+    for (uint32_t i = tid; i < (uint32_t)passes * kRadix; i += 256)
+      if (s_hist[i]) atomicAdd(&dst[i], s_hist[i]);
+  } else {
+    for (uint32_t i = tid; i < (uint32_t)passes * kRadix; i += 256) dst[i] = s_hist[i];
+  }
+}
+
+// Counts -> exclusive digit starts within the segment, in place, one
+// workgroup per segment; block 0 also resets the pass tickets.
+__global__ __launch_bounds__(256) void seg_scan_kernel(uint32_t* seg_cnt, int passes, SortCtrl* ctrl) {
+  __shared__ uint32_t s_wsum[4];
+  const uint32_t tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  uint32_t* c = seg_cnt + (uint64_t)blockIdx.x * passes * kRadix;
+  for (int p = 0; p < passes; ++p) {
+    const uint32_t x = c[p * kRadix + tid];
+    const uint32_t incl = wave_incl_scan(x);
+    if (lane == 63) s_wsum[wave] = incl;
+    __syncthreads();
+    uint32_t add = 0;
+    for (uint32_t w = 0; w < wave; ++w) add += s_wsum[w];
+    c[p * kRadix + tid] = incl - x + add;
+    __syncthreads();
+  }
+  if (blockIdx.x == 0 && tid < kMaxPasses) ctrl->tile_counter[tid] = 0;
+}
+
+struct SegLayout {
+  size_t off_keys_tmp, off_vals_tmp, off_ctrl, off_cnt, off_status, total;
+  int passes;
+};
+
+static SegLayout seg_layout(int kb, int vb, uint64_t n_total, uint64_t tiles, uint32_t nslots) {
+  SegLayout L{};
+  L.passes = kb;
+  size_t o = 0;
+  L.off_keys_tmp = o;
+  o = align_up(o + n_total * kb, 256);
+  L.off_vals_tmp = o;
+  o = align_up(o + n_total * vb, 256);
+  L.off_ctrl = o;
+  o = align_up(o + sizeof(SortCtrl), 256);
+  L.off_cnt = o;
+  o = align_up(o + (size_t)nslots * L.passes * kRadix * 4, 256);
+  L.off_status = o;
+  o = align_up(o + (size_t)L.passes * tiles * kRadix * 4, 256);
+  L.total = o;
+  return L;
+}
+
+size_t radix_segments_workspace_bytes(int key_bytes, int val_bytes, uint64_t n_total, uint64_t tiles,
+                                      uint32_t nslots) {
+  return seg_layout(key_bytes, val_bytes, n_total, tiles, nslots).total;
+}
+
+template <typename U, int VB, typename Idx>
+static hipError_t radix_segments_t(U* keys, void* vals_v, const SegTiles& T, const XformParams& xp,
+                                   unsigned char* ws, const SegLayout& L, hipStream_t stream, LaunchTimer* timer) {
+  using V = typename ValT<VB>::T;
+  constexpr bool KV = VB != 0;
+  constexpr int THREADS = Geometry<U, VB>::THREADS;
+  constexpr int ITEMS = Geometry<U, VB>::ITEMS;
+  const Xform<U> xf = xform_of<U>(xp);
+  const bool ident = xf.float_mask == 0 && xf.sign_flip == 0 && xf.desc_mask == 0;
+  SortCtrl* ctrl = reinterpret_cast<SortCtrl*>(ws + L.off_ctrl);
+  uint32_t* cnt = reinterpret_cast<uint32_t*>(ws + L.off_cnt);
+  uint32_t* status = reinterpret_cast<uint32_t*>(ws + L.off_status);
+  const uint64_t status_per_pass = T.tiles * kRadix;
+
+  if (T.n_multi) {
+    hipError_t e = fill_u32(cnt, (uint64_t)T.n_multi * L.passes * kRadix, 0u, stream);
+    if (e != hipSuccess) return e;
+  }
+  SegHistArgs<U> h;
+  h.keys = keys;
+  h.xf = xf;
+  h.passes = L.passes;
+  h.tile_keys = THREADS * ITEMS;
+  h.tile_seg = T.tile_seg;
+  h.seg_first = T.seg_first;
+  h.seg_range = T.seg_range;
+  h.n_multi = T.n_multi;
+  h.seg_cnt = cnt;
+  h.status = status;
+  h.status_per_pass = status_per_pass;
+  if (timer) timer->begin("seg_hist", T.n_keys * sizeof(U), stream);
+  hipLaunchKernelGGL(seg_hist_kernel<U>, dim3((uint32_t)T.tiles), dim3(256), 0, stream, h);
+  if (timer) timer->end(stream);
+  hipLaunchKernelGGL(seg_scan_kernel, dim3(T.nslots), dim3(256), 0, stream, cnt, L.passes, ctrl);
+  for (int p = 0; p < L.passes; ++p) {
+    OnesweepArgs<U, V, uint32_t> a{};
+    a.keys0 = keys;
+    a.keys1 = reinterpret_cast<U*>(ws + L.off_keys_tmp);
+    a.vals0 = reinterpret_cast<V*>(vals_v);
+    a.vals1 = reinterpret_cast<V*>(ws + L.off_vals_tmp);
+    a.n = 0;
+    a.xf = xf;
+    a.pass = p;
+    a.shift = 8 * p;
+    a.digit_base = nullptr;
+    a.status = status + p * status_per_pass;
+    a.ctrl = ctrl;
+    a.tiles = (uint32_t)T.tiles;
+    a.tile_seg = T.tile_seg;
+    a.seg_first = T.seg_first;
+    a.seg_range = T.seg_range;
+    a.seg_base = cnt;
+    a.passes = L.passes;
+    if (timer) timer->begin("seg_onesweep_pass", 2 * T.n_keys * (sizeof(U) + (KV ? sizeof(V) : 0)), stream);
+    if (ident)
+      hipLaunchKernelGGL((onesweep_kernel<U, V, KV, THREADS, ITEMS, uint32_t, true, true, Idx>),
+                         dim3((uint32_t)T.tiles), dim3(THREADS), 0, stream, a);
+    else
+      hipLaunchKernelGGL((onesweep_kernel<U, V, KV, THREADS, ITEMS, uint32_t, false, true, Idx>),
+                         dim3((uint32_t)T.tiles), dim3(THREADS), 0, stream, a);
+    if (timer) timer->end(stream);
+  }
+  return hipGetLastError();
+}
+
+template <typename U, int VB>
+static hipError_t radix_segments_i(U* keys, void* vals, const SegTiles& T, const XformParams& xp, unsigned char* ws,
+                                   const SegLayout& L, hipStream_t stream, LaunchTimer* timer) {
+  if (T.n_total < (1ull << 32)) return radix_segments_t<U, VB, uint32_t>(keys, vals, T, xp, ws, L, stream, timer);
+  return radix_segments_t<U, VB, uint64_t>(keys, vals, T, xp, ws, L, stream, timer);
+}
+
+hipError_t radix_sort_segments(int kb, int vb, void* keys, void* vals, const SegTiles& T, const XformParams& xf,
+                               void* workspace, size_t workspace_bytes, hipStream_t stream, LaunchTimer* timer) {
+  if (T.tiles == 0) return hipSuccess;
+  if (T.tiles >= (1ull << 31)) return hipErrorInvalidValue;
+  const SegLayout L = seg_layout(kb, vb, T.n_total, T.tiles, T.nslots);
+  if (workspace_bytes < L.total) return hipErrorInvalidValue;
+  unsigned char* ws = reinterpret_cast<unsigned char*>(workspace);
+  if (kb == 4) {
+    uint32_t* k = reinterpret_cast<uint32_t*>(keys);
+    if (vb == 0) return radix_segments_i<uint32_t, 0>(k, vals, T, xf, ws, L, stream, timer);
+    if (vb == 4) return radix_segments_i<uint32_t, 4>(k, vals, T, xf, ws, L, stream, timer);
+    if (vb == 8) return radix_segments_i<uint32_t, 8>(k, vals, T, xf, ws, L, stream, timer);
+  } else if (kb == 8) {
+    uint64_t* k = reinterpret_cast<uint64_t*>(keys);
+    if (vb == 0) return radix_segments_i<uint64_t, 0>(k, vals, T, xf, ws, L, stream, timer);
+    if (vb == 4) return radix_segments_i<uint64_t, 4>(k, vals, T, xf, ws, L, stream, timer);
+    if (vb == 8) return radix_segments_i<uint64_t, 8>(k, vals, T, xf, ws, L, stream, timer);
   }
   return hipErrorInvalidValue;
 }

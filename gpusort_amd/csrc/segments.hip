@@ -1,13 +1,24 @@
 // Batched segmented sort (gs_sort_segments).
 //
-// Segments are binned by size class on the host from the offsets:
-//   small  (<= one LDS tile): one workgroup per segment, bitonic in LDS, all
-//          small segments in one launch
-//   large  : onesweep radix sort per segment on the same stream
+// The reference sorts one array per call (CudaUtils::Sort / SortByKey,
+// src/cuda_utils.cu:97-243); a caller with many arrays pays one launch
+// sequence and one synchronisation per array.  Here every segment of one
+// call is sorted by a fixed number of launches, whatever the count:
+//
+//   small  (<= one LDS tile, block_sort_max): one workgroup per segment,
+//          bitonic network in LDS, all small segments in one launch
+//   radix  (up to 2^30 keys): segmented onesweep -- one histogram launch,
+//          one scan launch and one launch per digit for all of them; tiles
+//          never straddle segments and the look-back stops at the segment's
+//          first tile.  Multi-tile segments are numbered first.
+//   huge   (>= 2^30 keys): the single-array radix sort, one per segment
+//
 // Every class is stable with a payload, so each segment's result equals a
 // separate gs_sort_pairs call on it.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+#include <cstring>
 #include <vector>
 
 #include "../../include/gpusort_hip.h"
@@ -15,6 +26,17 @@
 #include "gs_internal.h"
 
 using namespace gs;
+
+namespace {
+
+constexpr uint64_t kSegRadixMax = 1ull << 30;
+
+struct Meta {
+  std::vector<unsigned char> buf;
+  size_t off_tile_seg = 0, off_seg_first = 0, off_seg_range = 0, off_small = 0;
+};
+
+}  // namespace
 
 extern "C" gs_status gs_sort_segments(gs_handle* h, gs_dtype kdt, void* d_keys, gs_dtype vdt, void* d_vals,
                                       const uint64_t* d_offsets, uint32_t nseg, int descending) {
@@ -33,34 +55,100 @@ extern "C" gs_status gs_sort_segments(gs_handle* h, gs_dtype kdt, void* d_keys, 
   for (uint32_t s = 0; s < nseg; ++s)
     if (off[s + 1] < off[s]) return fail(GS_INVALID_ARG, "segment offsets decrease at segment %u", s);
 
+  // Size classes.
   const uint32_t small_max = block_sort_max(kb, vb);
-  std::vector<uint32_t> small_ids;
-  std::vector<uint32_t> large_ids;
+  const uint64_t tile = radix_tile_keys(kb, vb);
+  std::vector<uint32_t> small_ids, multi_ids, single_ids, huge_ids;
   for (uint32_t s = 0; s < nseg; ++s) {
     const uint64_t len = off[s + 1] - off[s];
     if (len <= 1) continue;
     if (len <= small_max)
       small_ids.push_back(s);
+    else if (len >= kSegRadixMax)
+      huge_ids.push_back(s);
+    else if (len > tile)
+      multi_ids.push_back(s);
     else
-      large_ids.push_back(s);
+      single_ids.push_back(s);
   }
+
+  // Device maps of the radix class (multi-tile slots first) and the small ids,
+  // uploaded in one copy.
+  const uint32_t nslots = (uint32_t)(multi_ids.size() + single_ids.size());
+  uint64_t tiles = 0, n_keys = 0;
+  for (uint32_t s : multi_ids) tiles += (off[s + 1] - off[s] + tile - 1) / tile;
+  tiles += single_ids.size();
+  Meta m;
+  size_t o = 0;
+  m.off_tile_seg = o;
+  o = align_up(o + tiles * 4, 256);
+  m.off_seg_first = o;
+  o = align_up(o + (size_t)nslots * 4, 256);
+  m.off_seg_range = o;
+  o = align_up(o + (size_t)nslots * 16, 256);
+  m.off_small = o;
+  o = align_up(o + small_ids.size() * 4, 256);
+  m.buf.assign(o, 0);
+  {
+    uint32_t* tile_seg = reinterpret_cast<uint32_t*>(m.buf.data() + m.off_tile_seg);
+    uint32_t* seg_first = reinterpret_cast<uint32_t*>(m.buf.data() + m.off_seg_first);
+    uint64_t* seg_range = reinterpret_cast<uint64_t*>(m.buf.data() + m.off_seg_range);
+    uint64_t t = 0;
+    uint32_t slot = 0;
+    for (const std::vector<uint32_t>* ids : {&multi_ids, &single_ids}) {
+      for (uint32_t s : *ids) {
+        const uint64_t len = off[s + 1] - off[s];
+        const uint64_t nt = (len + tile - 1) / tile;
+        seg_first[slot] = (uint32_t)t;
+        seg_range[2 * slot] = off[s];
+        seg_range[2 * slot + 1] = off[s + 1];
+        for (uint64_t j = 0; j < nt; ++j) tile_seg[t++] = slot;
+        n_keys += len;
+        ++slot;
+      }
+    }
+    if (!small_ids.empty())
+      memcpy(m.buf.data() + m.off_small, small_ids.data(), small_ids.size() * 4);
+  }
+  unsigned char* dm = nullptr;
+  if (!small_ids.empty() || nslots) {
+    void* dmeta = nullptr;
+    e = h->arena.get(Arena::kSegAux, m.buf.size(), h->stream, &dmeta);
+    if (e != hipSuccess) return from_hip(e, "segment maps");
+    e = hipMemcpyAsync(dmeta, m.buf.data(), m.buf.size(), hipMemcpyHostToDevice, h->stream);
+    if (e != hipSuccess) return from_hip(e, "upload segment maps");
+    dm = reinterpret_cast<unsigned char*>(dmeta);
+  }
+
   if (!small_ids.empty()) {
-    void* ws = nullptr;
-    e = h->arena.get(Arena::kSegAux, small_ids.size() * sizeof(uint32_t), h->stream, &ws);
-    if (e != hipSuccess) return from_hip(e, "segment workspace");
-    e = hipMemcpyAsync(ws, small_ids.data(), small_ids.size() * sizeof(uint32_t), hipMemcpyHostToDevice, h->stream);
-    if (e == hipSuccess)
-      e = block_sort_segments(kb, vb, d_keys, d_vals, d_offsets, reinterpret_cast<const uint32_t*>(ws),
-                              (uint32_t)small_ids.size(), xf, h->stream);
+    e = block_sort_segments(kb, vb, d_keys, d_vals, d_offsets, reinterpret_cast<const uint32_t*>(dm + m.off_small),
+                            (uint32_t)small_ids.size(), xf, h->stream);
     if (e != hipSuccess) return from_hip(e, "small segments");
+  }
+  if (nslots) {
+    SegTiles T;
+    T.tile_seg = reinterpret_cast<const uint32_t*>(dm + m.off_tile_seg);
+    T.seg_first = reinterpret_cast<const uint32_t*>(dm + m.off_seg_first);
+    T.seg_range = reinterpret_cast<const uint64_t*>(dm + m.off_seg_range);
+    T.tiles = tiles;
+    T.nslots = nslots;
+    T.n_multi = (uint32_t)multi_ids.size();
+    T.n_total = off[nseg];
+    T.n_keys = n_keys;
+    const size_t ws_bytes = radix_segments_workspace_bytes(kb, vb, T.n_total, tiles, nslots);
+    void* ws = nullptr;
+    e = h->arena.get(Arena::kSegWork, ws_bytes, h->stream, &ws);
+    if (e != hipSuccess) return from_hip(e, "segmented radix workspace");
+    e = radix_sort_segments(kb, vb, d_keys, d_vals, T, xf, ws, ws_bytes, h->stream, h->timer());
+    if (e != hipSuccess) return from_hip(e, "segmented radix sort");
   }
   char* kbase = reinterpret_cast<char*>(d_keys);
   char* vbase = reinterpret_cast<char*>(d_vals);
-  for (uint32_t s : large_ids) {
+  for (uint32_t s : huge_ids) {
     const uint64_t len = off[s + 1] - off[s];
     gs_status st = sort_device(h, kdt, kbase + off[s] * kb, vdt, vb ? vbase + off[s] * vb : nullptr, len, descending);
     if (st != GS_OK) return st;
   }
-  // The id list above lives in host memory owned by this frame.
+  // The maps were copied from host memory owned by this frame.
   return from_hip(hipStreamSynchronize(h->stream), "segmented sort");
 }

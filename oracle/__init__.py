@@ -136,6 +136,26 @@ def stable_sort_pairs(k: np.ndarray, v: np.ndarray, descending: bool = False):
     return kk, vv
 
 
+def seg_sort(a: np.ndarray, offsets: np.ndarray, descending: bool = False) -> np.ndarray:
+    """Every segment [off[s], off[s+1]) sorted on its own (keys only)."""
+    out = np.ascontiguousarray(a).copy()
+    off = np.ascontiguousarray(offsets, dtype=np.int64)
+    fn = getattr(lib(), f"orc_seg_sort_{KEY_NAMES[out.dtype]}")
+    fn(_p(out), _p(off), ctypes.c_int64(off.size - 1), ctypes.c_int(int(descending)))
+    return out
+
+
+def seg_stable_sort_pairs(k: np.ndarray, v: np.ndarray, offsets: np.ndarray, descending: bool = False):
+    """Every segment stable-sorted by key with its payload (the batched form of
+    thrust::sort_by_key, cuda_utils.cu:477)."""
+    kk = np.ascontiguousarray(k).copy()
+    vv = np.ascontiguousarray(v).copy()
+    off = np.ascontiguousarray(offsets, dtype=np.int64)
+    fn = getattr(lib(), f"orc_seg_stable_sort_pairs_{KEY_NAMES[kk.dtype]}_{VAL_NAMES[vv.dtype]}")
+    fn(_p(kk), _p(vv), _p(off), ctypes.c_int64(off.size - 1), ctypes.c_int(int(descending)))
+    return kk, vv
+
+
 def merge(a: np.ndarray, b: np.ndarray, descending: bool = False) -> np.ndarray:
     """OmpUtils::Merge restated (omp_utils.h:54-145)."""
     a = np.ascontiguousarray(a)

@@ -525,6 +525,16 @@ int orc_get_threads(void) { return max_threads(); }
     else                                                                                         \
       std::sort(a, a + n, std::less<K>());                                                       \
   }                                                                                              \
+  /* Batched form: every segment [off[s], off[s+1]) sorted on its own (the  \
+     reference sorts one array per CudaUtils::Sort call). */                 \
+  void orc_seg_sort_##kn(K* a, const int64_t* off, int64_t nseg, int desc) {                     \
+    _Pragma("omp parallel for schedule(dynamic, 64)") for (int64_t s = 0; s < nseg; s++) {       \
+      if (desc)                                                                                  \
+        std::sort(a + off[s], a + off[s + 1], std::greater<K>());                                \
+      else                                                                                       \
+        std::sort(a + off[s], a + off[s + 1], std::less<K>());                                   \
+    }                                                                                            \
+  }                                                                                              \
   void orc_merge_##kn(const K* a, int64_t na, const K* b, int64_t nb, K* out, int desc) {        \
     if (desc)                                                                                    \
       omp_merge(std::greater<K>(), a, a + na, b, b + nb, out);                                   \
@@ -586,6 +596,11 @@ int orc_get_threads(void) { return max_threads(); }
     }                                                                                                  \
     std::copy(kk.begin(), kk.end(), k);                                                                \
     std::copy(vv.begin(), vv.end(), v);                                                                \
+  }                                                                                                    \
+  /* Batched form of the stable KV sort, one segment at a time. */                                     \
+  void orc_seg_stable_sort_pairs_##kn##_##vn(K* k, V* v, const int64_t* off, int64_t nseg, int desc) { \
+    _Pragma("omp parallel for schedule(dynamic, 64)") for (int64_t s = 0; s < nseg; s++)               \
+        orc_stable_sort_pairs_##kn##_##vn(k + off[s], v + off[s], off[s + 1] - off[s], desc);         \
   }                                                                                                    \
   int orc_vrank_pairs_##kn##_##vn(int algo, K* keys, V* vals, const int64_t* rank_off, int p,          \
                                   int k_way, int s, int desc, K* spl_k, V* spl_v, int64_t spl_cap,     \

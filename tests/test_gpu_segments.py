@@ -1,0 +1,111 @@
+"""Batched segmented sort (gs_sort_segments) against the oracle's per-segment
+sort, bit for bit: every size class (LDS tile, single radix tile, multi-tile
+segmented onesweep) and its boundaries, all key types, payload widths and both
+directions, skewed and constant keys, and a lognormal batch shaped like the
+batched-arrays workload."""
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+import oracle
+from gpusort_amd import datagen
+from gpu_util import assert_bits_equal, dev, host, say
+
+pytestmark = pytest.mark.gpu
+
+# Around every class boundary: LDS tile caps 2048/4096/8192, onesweep tiles
+# 6144 (64-bit keys or payloads) and 16384 (32-bit keys with 0/4-byte payloads).
+BOUNDARY_LENGTHS = [0, 1, 2, 3, 2047, 2048, 2049, 4095, 4096, 4097, 6143, 6144, 6145, 8191, 8192, 8193,
+                    12288, 12289, 16383, 16384, 16385, 32768, 32769, 1, 0, 50000, 100000, 262147, 5]
+
+
+def keys_of(dtype, n, seed, distinct=None):
+    d = np.dtype(dtype)
+    r = datagen.draws(seed, n)
+    if distinct:
+        r = datagen.draws(seed + 1, distinct)[r % np.uint64(distinct)]
+    if d.kind == "f":
+        x = (r >> np.uint64(40)).astype(np.float64) - 2.0**23
+        return x.astype(d)
+    if d.itemsize == 4:
+        return (r >> np.uint64(32)).astype(np.uint32).view(d)
+    return r.view(d)
+
+
+def offsets_of(lengths) -> np.ndarray:
+    return np.concatenate([[0], np.cumsum(np.asarray(lengths, dtype=np.uint64))]).astype(np.uint64)
+
+
+def run_case(handle, kdt, vdt, lengths, seed, descending, distinct=None):
+    off = offsets_of(lengths)
+    n = int(off[-1])
+    keys = keys_of(kdt, n, seed, distinct)
+    vals = None if vdt is None else np.arange(n, dtype=np.uint64).astype(vdt)
+    say(f"segments kdt={np.dtype(kdt).name} vdt={None if vdt is None else np.dtype(vdt).name} "
+        f"nseg={len(lengths)} n={n} desc={descending} seed={seed}")
+    tk = dev(keys)
+    tv = None if vals is None else dev(vals)
+    handle.sort_segments(tk, dev(off), tv, descending=descending)
+    off64 = off.astype(np.int64)
+    if vals is None:
+        want = oracle.seg_sort(keys, off64, descending)
+        assert_bits_equal(host(tk), want, "keys")
+    else:
+        wk, wv = oracle.seg_stable_sort_pairs(keys, vals, off64, descending)
+        assert_bits_equal(host(tk), wk, "keys")
+        assert_bits_equal(host(tv), wv, "payloads")
+
+
+TYPES = [
+    (np.uint32, None), (np.uint32, np.uint32), (np.int32, np.uint64), (np.float32, np.uint32),
+    (np.uint64, None), (np.uint64, np.uint32), (np.int64, np.uint64), (np.float64, np.uint32),
+]
+
+
+@pytest.mark.parametrize("kdt,vdt", TYPES,
+                         ids=lambda t: "none" if t is None else np.dtype(t).name)
+def test_boundary_lengths(handle, kdt, vdt):
+    run_case(handle, kdt, vdt, BOUNDARY_LENGTHS, 21, descending=False)
+
+
+@pytest.mark.parametrize("kdt,vdt", [(np.uint32, np.uint32), (np.int64, None), (np.float32, np.uint64)],
+                         ids=lambda t: "none" if t is None else np.dtype(t).name)
+def test_boundary_lengths_descending(handle, kdt, vdt):
+    run_case(handle, kdt, vdt, BOUNDARY_LENGTHS[::-1], 22, descending=True)
+
+
+@pytest.mark.parametrize("distinct", [1, 3, 200], ids=["equal", "three", "few"])
+def test_skewed_keys(handle, distinct):
+    # Ties everywhere: stability across the tiles of a segment is what is checked.
+    run_case(handle, np.uint32, np.uint32, BOUNDARY_LENGTHS, 23, descending=False, distinct=distinct)
+
+
+def test_lognormal_batch(handle):
+    lengths = datagen.lognormal_lengths(5, 3000, median=2048, sigma=2.0, lo=1, hi=1 << 20)
+    run_case(handle, np.uint32, np.uint32, lengths, 24, descending=False)
+
+
+def test_many_tiny_segments(handle):
+    lengths = datagen.lognormal_lengths(6, 40000, median=40, sigma=1.0, lo=0, hi=5000)
+    run_case(handle, np.uint64, np.uint32, lengths, 25, descending=True)
+
+
+def test_one_large_segment_equals_single_sort(handle):
+    # The segmented path over one segment must equal the single-array sort.
+    n = 3_000_017
+    keys = keys_of(np.uint32, n, 26)
+    vals = np.arange(n, dtype=np.uint32)
+    a_k, a_v = dev(keys), dev(vals)
+    b_k, b_v = dev(keys), dev(vals)
+    say(f"one segment n={n}")
+    handle.sort_segments(a_k, dev(offsets_of([n])), a_v)
+    handle.sort_pairs(b_k, b_v)
+    assert_bits_equal(host(a_k), host(b_k), "keys")
+    assert_bits_equal(host(a_v), host(b_v), "payloads")
+
+
+def test_repeat_calls_shrinking(handle):
+    # Workspace and maps are reused: a second, smaller call must not see the first.
+    run_case(handle, np.uint32, np.uint32, [70000, 20000, 5000, 100], 27, descending=False)
+    run_case(handle, np.uint32, np.uint32, [17000, 3], 28, descending=False)

@@ -251,3 +251,19 @@ def test_datagen_streams_are_counter_based():
     L = datagen.lognormal_lengths(1, 100_000)
     assert L.min() >= 1 and L.max() <= 1 << 22
     assert abs(math.log(np.median(L)) - math.log(2048)) < 0.05
+
+
+@pytest.mark.parametrize("descending", [False, True])
+def test_segmented_oracle_matches_lexsort(descending):
+    # Batched oracle = numpy's stable lexsort by (segment, key) (closed form).
+    rng = np.random.default_rng(7)
+    lengths = rng.integers(0, 300, 200)
+    off = np.concatenate([[0], np.cumsum(lengths)]).astype(np.int64)
+    n = int(off[-1])
+    k = rng.integers(-50, 50, n).astype(np.int64)
+    v = np.arange(n, dtype=np.uint32)
+    seg = np.repeat(np.arange(lengths.size), lengths)
+    order = np.lexsort((-k if descending else k, seg))
+    kk, vv = oracle.seg_stable_sort_pairs(k, v, off, descending)
+    assert np.array_equal(kk, k[order]) and np.array_equal(vv, v[order])
+    assert np.array_equal(oracle.seg_sort(k, off, descending), k[order])
