@@ -218,6 +218,106 @@ def run_batched(h, g, torch, n_arrays: int, steps: int, seed: int = 4):
     }
 
 
+def run_served(h, g, torch, n_requests: int, seed: int = 5, loads=(0.5, 0.9, 1.5)):
+    """Served sort requests through gs_server: sizes log-uniform over
+    [2^10, 2^28] keys; 50% u32 keys, 30% u64 keys + u32 payload, 20% f32 keys
+    ~ N(0, 1e3); device-resident inputs; Poisson arrivals.  The same requests
+    go through the serial server (one call per request, one at a time: the
+    one-call-per-array baseline) and the served one (4 stream slots, small
+    requests coalesced).  Latency = submission -> completion seen by the
+    server; every result is compared with the request sorted alone."""
+    from gpusort_amd import datagen
+
+    sizes = datagen.loguniform_sizes(seed, n_requests, lo=1 << 10, hi=1 << 28)
+    u = (datagen.draws(seed + 1, n_requests) >> np.uint64(11)).astype(np.float64) / float(1 << 53)
+    kinds = np.where(u < 0.5, 0, np.where(u < 0.8, 1, 2))  # 0 u32, 1 u64+u32, 2 f32
+    total = int(sizes.sum())
+
+    def fill(i, k, v):
+        if kinds[i] == 0:
+            h.generate(k, g.GEN_UNIFORM_U32, 1000 + i)
+        elif kinds[i] == 1:
+            h.generate(k, g.GEN_UNIFORM_U64, 1000 + i)
+            h.generate(v, g.GEN_IOTA_U32, 0)
+        else:
+            h.generate(k, g.GEN_NORMAL_F32, 1000 + i, param=1e3)
+
+    kdt = {0: torch.uint32, 1: torch.uint64, 2: torch.float32}
+    bufs = []
+    for i in range(n_requests):
+        n = int(sizes[i])
+        k = torch.empty(n, dtype=kdt[int(kinds[i])], device="cuda")
+        v = torch.empty(n, dtype=torch.uint32, device="cuda") if kinds[i] == 1 else None
+        bufs.append((k, v))
+    expect = []
+    for i, (k, v) in enumerate(bufs):
+        ek = torch.empty_like(k)
+        ev = None if v is None else torch.empty_like(v)
+        fill(i, ek, ev)
+        if ev is None:
+            h.sort_keys(ek)
+        else:
+            h.sort_pairs(ek, ev)
+        expect.append((ek, ev))
+    torch.cuda.synchronize()
+
+    def as_int(t):
+        return t.view(torch.int32) if t.element_size() == 4 else t.view(torch.int64)
+
+    def one_run(serial: bool, rate):
+        for i, (k, v) in enumerate(bufs):
+            fill(i, k, v)
+        torch.cuda.synchronize()
+        if rate is None:
+            arrivals = np.zeros(n_requests)
+        else:
+            gaps = -np.log(1.0 - (datagen.draws(seed + 2, n_requests) >> np.uint64(11)).astype(np.float64)
+                           / float(1 << 53)) / rate
+            arrivals = np.cumsum(gaps) - gaps[0]
+        srv = g.Server(serial=serial, streams=4, hbm_budget=200 * 10**9, coalesce_max_keys=1 << 16)
+        t0 = time.perf_counter()
+        ids = []
+        for i, (k, v) in enumerate(bufs):
+            while time.perf_counter() - t0 < arrivals[i]:
+                pass
+            ids.append(srv.submit(k, v))
+        srv.drain()
+        infos = [srv.info(r) for r in ids]
+        st = srv.stats()
+        srv.close()
+        lat = np.array([(x["done_us"] - x["submit_us"]) * 1e-3 for x in infos])
+        span_s = (max(x["done_us"] for x in infos) - min(x["submit_us"] for x in infos)) * 1e-6
+        exact = sum(bool(torch.equal(as_int(k), as_int(ek))) and (v is None or bool(torch.equal(as_int(v), as_int(ev))))
+                    for (k, v), (ek, ev) in zip(bufs, expect))
+        return {
+            "gkeys_s": round(total / span_s / 1e9, 3),
+            "p50_ms": round(float(np.percentile(lat, 50)), 3),
+            "p99_ms": round(float(np.percentile(lat, 99)), 3),
+            "makespan_ms": round(span_s * 1e3, 3),
+            "bit_exact": f"{exact}/{n_requests}",
+            "batches": int(st["batches"]),
+            "coalesced_requests": int(st["coalesced_requests"]),
+            "peak_pool_GB": round(st["peak_pooled_bytes"] / 1e9, 2),
+        }
+
+    out = {"requests": n_requests, "keys": total,
+           "mix": {"u32": int((kinds == 0).sum()), "u64+u32": int((kinds == 1).sum()), "f32": int((kinds == 2).sum())}}
+    serial_sat = one_run(True, None)
+    served_sat = one_run(False, None)
+    out["saturation"] = {"one_call": serial_sat, "served": served_sat,
+                         "speedup": round(served_sat["gkeys_s"] / serial_sat["gkeys_s"], 2)}
+    cap = n_requests / (serial_sat["makespan_ms"] * 1e-3)  # requests/s the serial server sustains
+    out["loads"] = []
+    for f in loads:
+        rate = f * cap
+        out["loads"].append({"offered_vs_one_call_capacity": f,
+                             "offered_gkeys_s": round(rate * total / n_requests / 1e9, 3),
+                             "one_call": one_run(True, rate), "served": one_run(False, rate)})
+    del bufs, expect
+    torch.cuda.empty_cache()
+    return out
+
+
 def roofline_of(prof: dict, name: str = "onesweep_pass"):
     st = prof.get(name)
     if not st or st["launches"] == 0:
@@ -326,6 +426,7 @@ def main() -> int:
     ap.add_argument("--no-extra", action="store_true", help="skip the other distributions")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--arrays", type=int, default=100000, help="arrays in the batched workload")
+    ap.add_argument("--requests", type=int, default=200, help="requests in the served workload")
     ap.add_argument("--cpu-n", type=int, default=1 << 28, help="CPU baseline sample size")
     ap.add_argument("--copy-calib", action="store_true",
                     help="only run plain copies of --n uint32 (counter calibration) and exit")
@@ -369,6 +470,8 @@ def main() -> int:
                                        "verified": r["verified"]}
         log(f"[bench] batched: {args.arrays} arrays, one call vs one call per array")
         extra["batched_arrays_kv_u32"] = run_batched(h, g, torch, args.arrays, 3)
+        log(f"[bench] served: {args.requests} requests, serial vs served, Poisson load sweep")
+        extra["served_mixed"] = run_served(h, g, torch, args.requests)
         log("[bench] KV f32 normal 2^27")
         r = run_workload(h, g, torch, "normal_f32", n // 2, 3, 1, payload=True, profile=False)
         extra["kv_f32_normal_2^27"] = {"gkeys_s": round(r["gkeys_s"], 3), "ms": round(r["ms_per_sort"], 3),

@@ -29,6 +29,7 @@ STATUS_NAMES = {
     5: "GS_NO_DEVICE",
     6: "GS_UNSUPPORTED",
     7: "GS_EMPTY_INPUT",
+    8: "GS_NOT_READY",
 }
 
 _NP_DTYPE = {
@@ -81,7 +82,52 @@ GEN_UNIFORM_U32, GEN_UNIFORM_U64, GEN_ZIPF_U32, GEN_SORTED_U32, GEN_EQUAL_U32, G
 _lib = None
 
 # Exported C symbols and their argument types.
+class ServerConfig(ctypes.Structure):
+    _fields_ = [
+        ("device", ctypes.c_int),
+        ("hbm_budget", ctypes.c_uint64),
+        ("streams", ctypes.c_uint32),
+        ("coalesce_max_keys", ctypes.c_uint64),
+        ("batch_max_keys", ctypes.c_uint64),
+        ("batch_max_requests", ctypes.c_uint32),
+        ("serial", ctypes.c_int),
+    ]
+
+
+class RequestInfo(ctypes.Structure):
+    _fields_ = [
+        ("id", ctypes.c_uint64),
+        ("n", ctypes.c_uint64),
+        ("kdt", ctypes.c_int),
+        ("vdt", ctypes.c_int),
+        ("status", ctypes.c_int),
+        ("done", ctypes.c_int),
+        ("submit_us", ctypes.c_double),
+        ("start_us", ctypes.c_double),
+        ("done_us", ctypes.c_double),
+        ("batch", ctypes.c_uint64),
+        ("batch_size", ctypes.c_uint32),
+    ]
+
+
+class ServerStats(ctypes.Structure):
+    _fields_ = [(name, ctypes.c_uint64) for name in (
+        "submitted", "completed", "failed", "refused", "batches", "coalesced_batches", "coalesced_requests",
+        "pooled_bytes", "peak_pooled_bytes", "budget")]
+
+
 _SIGS = {
+    "gs_server_create": ([ctypes.POINTER(ctypes.c_void_p), ctypes.POINTER(ServerConfig)], ctypes.c_int),
+    "gs_server_submit": (
+        [ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p, ctypes.c_uint64, ctypes.c_int,
+         ctypes.POINTER(ctypes.c_uint64)],
+        ctypes.c_int,
+    ),
+    "gs_server_wait": ([ctypes.c_void_p, ctypes.c_uint64, ctypes.c_int64], ctypes.c_int),
+    "gs_server_drain": ([ctypes.c_void_p], ctypes.c_int),
+    "gs_server_info": ([ctypes.c_void_p, ctypes.c_uint64, ctypes.POINTER(RequestInfo)], ctypes.c_int),
+    "gs_server_get_stats": ([ctypes.c_void_p, ctypes.POINTER(ServerStats)], ctypes.c_int),
+    "gs_server_destroy": ([ctypes.c_void_p], ctypes.c_int),
     "gs_version": ([], ctypes.c_char_p),
     "gs_last_error": ([], ctypes.c_char_p),
     "gs_get_device_count": ([ctypes.POINTER(ctypes.c_int)], ctypes.c_int),
@@ -386,6 +432,70 @@ class Handle:
                 int(descending), None if pairs_out is None else pairs_out.ctypes.data, pair_stride, val_offset,
             )
         )
+
+
+class Server:
+    """Sort server over device-resident requests (gs_server_*).  Submitted
+    tensors belong to the server until their request is done."""
+
+    def __init__(self, device: int = 0, hbm_budget: int = 200 * 10**9, streams: int = 4,
+                 coalesce_max_keys: int = 1 << 16, batch_max_keys: int = 1 << 24, batch_max_requests: int = 4096,
+                 serial: bool = False):
+        self.lib = load_library()
+        cfg = ServerConfig(device, hbm_budget, streams, coalesce_max_keys, batch_max_keys, batch_max_requests,
+                           int(serial))
+        p = ctypes.c_void_p()
+        _check(self.lib.gs_server_create(ctypes.byref(p), ctypes.byref(cfg)))
+        self.s = p.value
+        self._live = {}
+
+    def submit(self, keys, vals=None, descending: bool = False) -> int:
+        rid = ctypes.c_uint64()
+        _check(self.lib.gs_server_submit(
+            self.s, torch_dtype_code(keys), _dev_ptr(keys), NONE if vals is None else torch_dtype_code(vals),
+            None if vals is None else _dev_ptr(vals), keys.numel(), int(descending), ctypes.byref(rid)))
+        self._live[rid.value] = (keys, vals)  # keep the buffers alive until done
+        return rid.value
+
+    def wait(self, rid: int, timeout_us: int = -1) -> bool:
+        st = self.lib.gs_server_wait(self.s, rid, timeout_us)
+        if st == 8:
+            return False
+        _check(st)
+        self._live.pop(rid, None)
+        return True
+
+    def drain(self) -> None:
+        _check(self.lib.gs_server_drain(self.s))
+        self._live.clear()
+
+    def info(self, rid: int) -> dict:
+        r = RequestInfo()
+        _check(self.lib.gs_server_info(self.s, rid, ctypes.byref(r)))
+        return {k: getattr(r, k) for k, _ in RequestInfo._fields_}
+
+    def stats(self) -> dict:
+        st = ServerStats()
+        _check(self.lib.gs_server_get_stats(self.s, ctypes.byref(st)))
+        return {k: getattr(st, k) for k, _ in ServerStats._fields_}
+
+    def close(self) -> None:
+        if getattr(self, "s", None):
+            self.lib.gs_server_destroy(self.s)
+            self.s = None
+            self._live.clear()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 def rebalance_sizes(sizes) -> np.ndarray:

@@ -57,7 +57,7 @@ hipError_t Arena::get(int slot, size_t bytes, hipStream_t stream, void** out) {
       ptr_[slot] = nullptr;
       cap_[slot] = 0;
     }
-    const size_t want = align_up(bytes + bytes / 8, 1 << 20);
+    const size_t want = grown(bytes);
     hipError_t e = hipMalloc(&ptr_[slot], want);
     if (e != hipSuccess) {
       (void)hipGetLastError();

@@ -50,6 +50,13 @@ class Arena {
   }
   ~Arena();
   hipError_t get(int slot, size_t bytes, hipStream_t stream, void** out);
+  // Bytes a slot allocates when it must grow to hold `bytes` (1/8 headroom,
+  // whole MiB).
+  static size_t grown(size_t bytes) {
+    if (bytes == 0) bytes = 256;
+    const size_t b = bytes + bytes / 8;
+    return (b + (1u << 20) - 1) & ~(size_t)((1u << 20) - 1);
+  }
   size_t bytes() const;
   void release();
 
@@ -75,6 +82,12 @@ int dtype_bytes(gs_dtype d);
 int val_bytes_of(gs_dtype d);
 gs_status sort_device(gs_handle* h, gs_dtype kdt, void* d_keys, gs_dtype vdt, void* d_vals, uint64_t n,
                       int descending);
+// Segmented sort from host offsets, queued on h->stream without waiting;
+// `meta` must outlive the queued work; segments of 2^30+ keys are returned
+// in `huge` for the caller to sort.
+gs_status sort_segments_async(gs_handle* h, gs_dtype kdt, void* d_keys, gs_dtype vdt, void* d_vals,
+                              const uint64_t* off, uint32_t nseg, int descending, std::vector<unsigned char>* meta,
+                              std::vector<uint32_t>* huge);
 
 }  // namespace gs
 

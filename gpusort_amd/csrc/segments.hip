@@ -27,72 +27,63 @@
 
 using namespace gs;
 
-namespace {
+namespace gs {
 
 constexpr uint64_t kSegRadixMax = 1ull << 30;
 
-struct Meta {
-  std::vector<unsigned char> buf;
-  size_t off_tile_seg = 0, off_seg_first = 0, off_seg_range = 0, off_small = 0;
-};
-
-}  // namespace
-
-extern "C" gs_status gs_sort_segments(gs_handle* h, gs_dtype kdt, void* d_keys, gs_dtype vdt, void* d_vals,
-                                      const uint64_t* d_offsets, uint32_t nseg, int descending) {
-  if (!h) return fail(GS_INVALID_ARG, "null handle");
-  if (!dtype_ok(kdt)) return fail(GS_INVALID_ARG, "unsupported key type %d", (int)kdt);
-  if (vdt != GS_NONE && !dtype_ok(vdt)) return fail(GS_INVALID_ARG, "unsupported value type %d", (int)vdt);
-  if (nseg == 0) return GS_OK;
-  if (!d_offsets || !d_keys || (vdt != GS_NONE && !d_vals)) return fail(GS_INVALID_ARG, "null pointer");
-  DeviceScope ds(h->device);
+// Sorts segment s = [off[s], off[s+1]) of keys/vals (host offsets) with no
+// host synchronisation: the launches are queued on h->stream.  The device
+// maps are uploaded from `meta`, which the caller keeps alive until the
+// stream has passed this point.  Segments of 2^30 keys or more, which the
+// segmented radix cannot take, are left to the caller (returned in *huge).
+gs_status sort_segments_async(gs_handle* h, gs_dtype kdt, void* d_keys, gs_dtype vdt, void* d_vals,
+                              const uint64_t* off, uint32_t nseg, int descending, std::vector<unsigned char>* meta,
+                              std::vector<uint32_t>* huge) {
   const int kb = dtype_bytes(kdt), vb = val_bytes_of(vdt);
   const XformParams xf = make_xform((int)kdt, descending != 0);
-  std::vector<uint64_t> off(nseg + 1);
-  hipError_t e = hipMemcpyAsync(off.data(), d_offsets, (nseg + 1) * sizeof(uint64_t), hipMemcpyDeviceToHost, h->stream);
-  if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
-  if (e != hipSuccess) return from_hip(e, "read segment offsets");
-  for (uint32_t s = 0; s < nseg; ++s)
-    if (off[s + 1] < off[s]) return fail(GS_INVALID_ARG, "segment offsets decrease at segment %u", s);
+  hipError_t e = hipSuccess;
 
   // Size classes.
   const uint32_t small_max = block_sort_max(kb, vb);
   const uint64_t tile = radix_tile_keys(kb, vb);
-  std::vector<uint32_t> small_ids, multi_ids, single_ids, huge_ids;
+  std::vector<uint32_t> small_ids, multi_ids, single_ids;
+  huge->clear();
   for (uint32_t s = 0; s < nseg; ++s) {
     const uint64_t len = off[s + 1] - off[s];
     if (len <= 1) continue;
     if (len <= small_max)
       small_ids.push_back(s);
     else if (len >= kSegRadixMax)
-      huge_ids.push_back(s);
+      huge->push_back(s);
     else if (len > tile)
       multi_ids.push_back(s);
     else
       single_ids.push_back(s);
   }
 
-  // Device maps of the radix class (multi-tile slots first) and the small ids,
-  // uploaded in one copy.
+  // Device maps of the radix class (multi-tile slots first), the small ids
+  // and the offsets, uploaded in one copy.
   const uint32_t nslots = (uint32_t)(multi_ids.size() + single_ids.size());
   uint64_t tiles = 0, n_keys = 0;
   for (uint32_t s : multi_ids) tiles += (off[s + 1] - off[s] + tile - 1) / tile;
   tiles += single_ids.size();
-  Meta m;
   size_t o = 0;
-  m.off_tile_seg = o;
+  const size_t off_tile_seg = o;
   o = align_up(o + tiles * 4, 256);
-  m.off_seg_first = o;
+  const size_t off_seg_first = o;
   o = align_up(o + (size_t)nslots * 4, 256);
-  m.off_seg_range = o;
+  const size_t off_seg_range = o;
   o = align_up(o + (size_t)nslots * 16, 256);
-  m.off_small = o;
+  const size_t off_small = o;
   o = align_up(o + small_ids.size() * 4, 256);
-  m.buf.assign(o, 0);
+  const size_t off_offsets = o;
+  o = align_up(o + (size_t)(nseg + 1) * 8, 256);
+  meta->assign(o, 0);
   {
-    uint32_t* tile_seg = reinterpret_cast<uint32_t*>(m.buf.data() + m.off_tile_seg);
-    uint32_t* seg_first = reinterpret_cast<uint32_t*>(m.buf.data() + m.off_seg_first);
-    uint64_t* seg_range = reinterpret_cast<uint64_t*>(m.buf.data() + m.off_seg_range);
+    unsigned char* b = meta->data();
+    uint32_t* tile_seg = reinterpret_cast<uint32_t*>(b + off_tile_seg);
+    uint32_t* seg_first = reinterpret_cast<uint32_t*>(b + off_seg_first);
+    uint64_t* seg_range = reinterpret_cast<uint64_t*>(b + off_seg_range);
     uint64_t t = 0;
     uint32_t slot = 0;
     for (const std::vector<uint32_t>* ids : {&multi_ids, &single_ids}) {
@@ -107,29 +98,28 @@ extern "C" gs_status gs_sort_segments(gs_handle* h, gs_dtype kdt, void* d_keys, 
         ++slot;
       }
     }
-    if (!small_ids.empty())
-      memcpy(m.buf.data() + m.off_small, small_ids.data(), small_ids.size() * 4);
+    if (!small_ids.empty()) memcpy(b + off_small, small_ids.data(), small_ids.size() * 4);
+    memcpy(b + off_offsets, off, (size_t)(nseg + 1) * 8);
   }
-  unsigned char* dm = nullptr;
-  if (!small_ids.empty() || nslots) {
-    void* dmeta = nullptr;
-    e = h->arena.get(Arena::kSegAux, m.buf.size(), h->stream, &dmeta);
-    if (e != hipSuccess) return from_hip(e, "segment maps");
-    e = hipMemcpyAsync(dmeta, m.buf.data(), m.buf.size(), hipMemcpyHostToDevice, h->stream);
-    if (e != hipSuccess) return from_hip(e, "upload segment maps");
-    dm = reinterpret_cast<unsigned char*>(dmeta);
-  }
+  if (small_ids.empty() && nslots == 0) return GS_OK;
+  void* dmeta = nullptr;
+  e = h->arena.get(Arena::kSegAux, meta->size(), h->stream, &dmeta);
+  if (e != hipSuccess) return from_hip(e, "segment maps");
+  e = hipMemcpyAsync(dmeta, meta->data(), meta->size(), hipMemcpyHostToDevice, h->stream);
+  if (e != hipSuccess) return from_hip(e, "upload segment maps");
+  unsigned char* dm = reinterpret_cast<unsigned char*>(dmeta);
 
   if (!small_ids.empty()) {
-    e = block_sort_segments(kb, vb, d_keys, d_vals, d_offsets, reinterpret_cast<const uint32_t*>(dm + m.off_small),
-                            (uint32_t)small_ids.size(), xf, h->stream);
+    e = block_sort_segments(kb, vb, d_keys, d_vals, reinterpret_cast<const uint64_t*>(dm + off_offsets),
+                            reinterpret_cast<const uint32_t*>(dm + off_small), (uint32_t)small_ids.size(), xf,
+                            h->stream);
     if (e != hipSuccess) return from_hip(e, "small segments");
   }
   if (nslots) {
     SegTiles T;
-    T.tile_seg = reinterpret_cast<const uint32_t*>(dm + m.off_tile_seg);
-    T.seg_first = reinterpret_cast<const uint32_t*>(dm + m.off_seg_first);
-    T.seg_range = reinterpret_cast<const uint64_t*>(dm + m.off_seg_range);
+    T.tile_seg = reinterpret_cast<const uint32_t*>(dm + off_tile_seg);
+    T.seg_first = reinterpret_cast<const uint32_t*>(dm + off_seg_first);
+    T.seg_range = reinterpret_cast<const uint64_t*>(dm + off_seg_range);
     T.tiles = tiles;
     T.nslots = nslots;
     T.n_multi = (uint32_t)multi_ids.size();
@@ -142,11 +132,35 @@ extern "C" gs_status gs_sort_segments(gs_handle* h, gs_dtype kdt, void* d_keys, 
     e = radix_sort_segments(kb, vb, d_keys, d_vals, T, xf, ws, ws_bytes, h->stream, h->timer());
     if (e != hipSuccess) return from_hip(e, "segmented radix sort");
   }
+  return GS_OK;
+}
+
+}  // namespace gs
+
+extern "C" gs_status gs_sort_segments(gs_handle* h, gs_dtype kdt, void* d_keys, gs_dtype vdt, void* d_vals,
+                                      const uint64_t* d_offsets, uint32_t nseg, int descending) {
+  if (!h) return fail(GS_INVALID_ARG, "null handle");
+  if (!dtype_ok(kdt)) return fail(GS_INVALID_ARG, "unsupported key type %d", (int)kdt);
+  if (vdt != GS_NONE && !dtype_ok(vdt)) return fail(GS_INVALID_ARG, "unsupported value type %d", (int)vdt);
+  if (nseg == 0) return GS_OK;
+  if (!d_offsets || !d_keys || (vdt != GS_NONE && !d_vals)) return fail(GS_INVALID_ARG, "null pointer");
+  DeviceScope ds(h->device);
+  const int kb = dtype_bytes(kdt), vb = val_bytes_of(vdt);
+  std::vector<uint64_t> off(nseg + 1);
+  hipError_t e = hipMemcpyAsync(off.data(), d_offsets, (nseg + 1) * sizeof(uint64_t), hipMemcpyDeviceToHost, h->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
+  if (e != hipSuccess) return from_hip(e, "read segment offsets");
+  for (uint32_t s = 0; s < nseg; ++s)
+    if (off[s + 1] < off[s]) return fail(GS_INVALID_ARG, "segment offsets decrease at segment %u", s);
+  std::vector<unsigned char> meta;
+  std::vector<uint32_t> huge;
+  gs_status st = sort_segments_async(h, kdt, d_keys, vdt, d_vals, off.data(), nseg, descending, &meta, &huge);
+  if (st != GS_OK) return st;
   char* kbase = reinterpret_cast<char*>(d_keys);
   char* vbase = reinterpret_cast<char*>(d_vals);
-  for (uint32_t s : huge_ids) {
+  for (uint32_t s : huge) {
     const uint64_t len = off[s + 1] - off[s];
-    gs_status st = sort_device(h, kdt, kbase + off[s] * kb, vdt, vb ? vbase + off[s] * vb : nullptr, len, descending);
+    st = sort_device(h, kdt, kbase + off[s] * kb, vdt, vb ? vbase + off[s] * vb : nullptr, len, descending);
     if (st != GS_OK) return st;
   }
   // The maps were copied from host memory owned by this frame.

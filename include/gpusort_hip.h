@@ -30,7 +30,8 @@ typedef enum gs_status {
   GS_DEVICE_ERROR = 4,
   GS_NO_DEVICE = 5,
   GS_UNSUPPORTED = 6,
-  GS_EMPTY_INPUT = 7
+  GS_EMPTY_INPUT = 7,
+  GS_NOT_READY = 8 /* a wait timed out before the request finished */
 } gs_status;
 
 /* Key / value element types: the primitive types of the reference's
@@ -208,6 +209,59 @@ gs_status gs_gather_host(gs_handle* h, int elem_bytes, const void* h_src, uint64
  * (include/par/par_utils.h:103-175): empty ranks take data from their
  * nearest non-empty neighbour; the data keeps its global order.  Host only. */
 gs_status gs_rebalance_sizes(uint64_t* sizes, uint32_t p);
+
+/* ------------------------------------------------------------------ serving
+ * A sort server: continuous batching of device-resident sort requests on one
+ * GPU.  No counterpart in the reference, whose callers each run their own
+ * sort call (src/cuda_utils.cu:97-243) with per-call device allocation.
+ * N stream slots with pooled HBM workspace; a scheduler thread admits
+ * requests in arrival order when a slot is free and the batch's workspace
+ * fits the HBM budget, coalesces small requests of one (key type, payload
+ * type, direction) into one segmented launch, runs larger ones on their own
+ * slot concurrently, and completes requests by event.  A request's buffers
+ * belong to the server until it is done; its result equals gs_sort_keys /
+ * gs_sort_pairs on the same input.  serial = 1 gives the one-call-per-request
+ * baseline (one slot, no coalescing).  Calls are thread-safe. */
+typedef struct gs_server gs_server;
+
+typedef struct gs_server_config {
+  int device;
+  uint64_t hbm_budget;          /* bytes of pooled workspace the server may hold */
+  uint32_t streams;             /* stream slots, 1..8 */
+  uint64_t coalesce_max_keys;   /* requests of at most this many keys are coalesced */
+  uint64_t batch_max_keys;      /* keys per coalesced launch (0: 2^24) */
+  uint32_t batch_max_requests;  /* requests per coalesced launch (0: 4096) */
+  int serial;                   /* 1: one request at a time, no coalescing */
+} gs_server_config;
+
+typedef struct gs_request_info {
+  uint64_t id, n;
+  gs_dtype kdt, vdt;
+  gs_status status;
+  int done;
+  double submit_us, start_us, done_us; /* server clock: submission, launch, completion seen */
+  uint64_t batch;                      /* launch the request ran in */
+  uint32_t batch_size;                 /* requests in that launch */
+} gs_request_info;
+
+typedef struct gs_server_stats {
+  uint64_t submitted, completed, failed, refused;
+  uint64_t batches, coalesced_batches, coalesced_requests;
+  uint64_t pooled_bytes, peak_pooled_bytes, budget;
+} gs_server_stats;
+
+gs_status gs_server_create(gs_server** out, const gs_server_config* cfg);
+/* Queues a request; GS_ADMISSION_REFUSED when its workspace alone exceeds the budget. */
+gs_status gs_server_submit(gs_server* s, gs_dtype kdt, void* d_keys, gs_dtype vdt, void* d_vals, uint64_t n,
+                           int descending, uint64_t* id);
+/* Waits for one request (timeout_us < 0: no limit); GS_NOT_READY on timeout. */
+gs_status gs_server_wait(gs_server* s, uint64_t id, int64_t timeout_us);
+/* Waits for every submitted request. */
+gs_status gs_server_drain(gs_server* s);
+gs_status gs_server_info(gs_server* s, uint64_t id, gs_request_info* out);
+gs_status gs_server_get_stats(gs_server* s, gs_server_stats* out);
+/* Finishes the queued work, then frees the server. */
+gs_status gs_server_destroy(gs_server* s);
 
 #ifdef __cplusplus
 }

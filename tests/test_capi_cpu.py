@@ -70,3 +70,14 @@ def test_argument_validation_without_device():
     assert lib.gs_vrank_sort(None, 0, g.U32, None, g.NONE, None, None, 0, 8, 32, 0, None) == 1
     assert lib.gs_gather(None, 4, None, None, None, 0) == 1
     assert lib.gs_profile_end(None, None, 0, None) == 1
+
+
+def test_server_needs_a_device(gpu_available):
+    if gpu_available:
+        pytest.skip("a device is visible")
+    with pytest.raises(g.GpusortError) as e:
+        g.Server()
+    assert e.value.status == 5
+    lib = g.load_library()
+    assert lib.gs_server_submit(None, g.U32, None, g.NONE, None, 0, 0, None) == 1
+    assert lib.gs_server_destroy(None) == 0

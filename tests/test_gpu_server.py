@@ -1,0 +1,117 @@
+"""Sort server (gs_server_*): every request's result equals the same input
+sorted alone by gs_sort_keys / gs_sort_pairs, whether it ran coalesced with
+other small requests, concurrently with large ones, or in the serial
+one-call-per-request mode; admission respects the HBM budget."""
+from __future__ import annotations
+
+import numpy as np
+import pytest
+import torch
+
+import gpusort_amd as g
+from gpusort_amd import datagen
+from gpu_util import say
+
+pytestmark = pytest.mark.gpu
+
+KINDS = [("u32", None), ("u64", torch.uint32), ("f32", None), ("u32", torch.uint32)]
+
+
+def make_request(h, kind, n, seed):
+    kname, vdt = kind
+    if kname == "u32":
+        k = torch.empty(n, dtype=torch.uint32, device="cuda")
+        h.generate(k, g.GEN_UNIFORM_U32, seed)
+    elif kname == "u64":
+        k = torch.empty(n, dtype=torch.uint64, device="cuda")
+        h.generate(k, g.GEN_UNIFORM_U64, seed)
+    else:
+        k = torch.empty(n, dtype=torch.float32, device="cuda")
+        h.generate(k, g.GEN_NORMAL_F32, seed, param=1e3)
+    v = None
+    if vdt is not None:
+        v = torch.empty(n, dtype=vdt, device="cuda")
+        h.generate(v, g.GEN_IOTA_U32, 0)
+    return k, v
+
+
+def sizes_mix(seed, count):
+    # log-uniform 2^4 .. 2^22, plus edge sizes
+    s = datagen.loguniform_sizes(seed, count, lo=1 << 4, hi=1 << 22).tolist()
+    return s + [0, 1, 2, 4096, 4097, 16384, 16385, 65536, 65537]
+
+
+def same(a, b) -> bool:
+    if a is None:
+        return b is None
+    ia = a.view(torch.int32) if a.element_size() == 4 else a.view(torch.int64)
+    ib = b.view(torch.int32) if b.element_size() == 4 else b.view(torch.int64)
+    return bool(torch.equal(ia, ib))
+
+
+def run_mix(handle, server, seed, descending_every=5):
+    sizes = sizes_mix(seed, 60)
+    reqs = []
+    for i, n in enumerate(sizes):
+        kind = KINDS[i % len(KINDS)]
+        desc = (i % descending_every) == 0
+        k, v = make_request(handle, kind, int(n), seed * 1000 + i)
+        reqs.append((kind, int(n), desc, k, v))
+    torch.cuda.synchronize()
+    say(f"server mix seed={seed} requests={len(reqs)} keys={sum(r[1] for r in reqs)}")
+    ids = [server.submit(k, v, desc) for (_, _, desc, k, v) in reqs]
+    server.drain()
+    for rid, (kind, n, desc, k, v) in zip(ids, reqs):
+        info = server.info(rid)
+        assert info["done"] == 1 and info["status"] == 0
+        assert info["submit_us"] <= info["done_us"]
+        k0, v0 = make_request(handle, kind, n, seed * 1000 + ids.index(rid))
+        if v0 is None:
+            handle.sort_keys(k0, descending=desc)
+        else:
+            handle.sort_pairs(k0, v0, descending=desc)
+        torch.cuda.synchronize()
+        assert same(k, k0), f"request {rid} ({kind}, n={n}, desc={desc}) keys differ from the alone sort"
+        assert same(v, v0), f"request {rid} payloads differ"
+    return server.stats()
+
+
+def test_served_mix_equals_alone(handle):
+    with g.Server(streams=4, coalesce_max_keys=1 << 16) as srv:
+        st = run_mix(handle, srv, 1)
+    assert st["completed"] == st["submitted"] and st["failed"] == 0
+    assert st["coalesced_batches"] > 0 and st["batches"] < st["submitted"]
+    assert st["peak_pooled_bytes"] <= st["budget"]
+
+
+def test_serial_mode_equals_alone(handle):
+    with g.Server(serial=True) as srv:
+        st = run_mix(handle, srv, 2)
+    assert st["coalesced_batches"] == 0
+    assert st["batches"] == st["submitted"] - 2  # sizes 0 and 1 complete at submission
+
+
+def test_tight_budget_still_completes(handle):
+    # Room for the workspace of one 2^22-key KV sort at a time.
+    budget = 80 << 20
+    with g.Server(streams=4, hbm_budget=budget) as srv:
+        st = run_mix(handle, srv, 3)
+        assert st["peak_pooled_bytes"] <= budget
+        big = torch.empty(1 << 26, dtype=torch.uint32, device="cuda")
+        with pytest.raises(g.GpusortError) as e:
+            srv.submit(big, torch.empty_like(big))
+        assert e.value.status == 3  # GS_ADMISSION_REFUSED
+        assert srv.stats()["refused"] == 1
+
+
+def test_wait_and_timeout(handle):
+    with g.Server(streams=2) as srv:
+        k, v = make_request(handle, ("u32", torch.uint32), 1 << 20, 77)
+        rid = srv.submit(k, v)
+        assert srv.wait(rid, timeout_us=-1)
+        info = srv.info(rid)
+        assert info["submit_us"] <= info["start_us"] <= info["done_us"]
+        k0, v0 = make_request(handle, ("u32", torch.uint32), 1 << 20, 77)
+        handle.sort_pairs(k0, v0)
+        torch.cuda.synchronize()
+        assert same(k, k0) and same(v, v0)
