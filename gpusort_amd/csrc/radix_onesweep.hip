@@ -415,17 +415,18 @@ __global__ __launch_bounds__(256) void radix_final_copy_kernel(U* __restrict__ k
 
 template <typename U, int VB>
 struct Geometry;
-// Tile sizes: 32-bit keys with no or 32-bit payloads use 1024 threads x 16
-// keys (16K-key tiles: fewer look-back steps per key), the rest 512 x 12.
+// Tile sizes: 32-bit keys with no or 32-bit payloads use 1024 threads x 18
+// keys (18K-key tiles: fewer look-back steps per key; 18 keys is the most
+// that fits 128 VGPRs without spills), the rest 512 x 12.
 // LDS holds one tile of the wider of key and payload, reused for the payload
 // scatter.
 template <>
 struct Geometry<uint32_t, 0> {
-  static constexpr int THREADS = 1024, ITEMS = 16;
+  static constexpr int THREADS = 1024, ITEMS = 18;
 };
 template <>
 struct Geometry<uint32_t, 4> {
-  static constexpr int THREADS = 1024, ITEMS = 16;
+  static constexpr int THREADS = 1024, ITEMS = 18;
 };
 template <>
 struct Geometry<uint32_t, 8> {
