@@ -318,6 +318,44 @@ def run_served(h, g, torch, n_requests: int, seed: int = 5, loads=(0.5, 0.9, 1.5
     return out
 
 
+def run_multisplit(h, g, torch, n: int, nspl: int = 63, steps: int = 5):
+    """Stable multisplit of 2^28 uniform u32 keys + u32 payloads into 64
+    buckets (splitters = evenly spaced sorted sample of the keys)."""
+    keys, vals = make_input(h, g, torch, "uniform", n, SEED)
+    sample = keys[:: max(1, n // 4096)].cpu().numpy()
+    spl = np.sort(sample)[np.linspace(0, sample.size - 1, nspl + 2).astype(np.int64)[1:-1]]
+    ok, ov = torch.empty_like(keys), torch.empty_like(vals)
+    h.multisplit(keys, spl, ok, vals, ov)
+    torch.cuda.synchronize()
+    h.profile_begin()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        offs = h.multisplit(keys, spl, ok, vals, ov)
+    t1 = time.perf_counter()
+    prof = h.profile_end()
+    # checks: buckets hold their ranges, payloads map input to output
+    idx = as_i64(torch, ov)
+    kk = as_i64(torch, ok)
+    ok_perm = bool((torch.bincount(idx, minlength=n) == 1).all()) and bool((as_i64(torch, keys)[idx] == kk).all())
+    b = torch.from_numpy(offs.astype(np.int64)).cuda()
+    seg = torch.repeat_interleave(torch.arange(nspl + 1, device="cuda"), b[1:] - b[:-1])
+    spl_t = torch.from_numpy(spl.astype(np.int64)).cuda()
+    lo = torch.cat([torch.tensor([-1], device="cuda"), spl_t])[seg]
+    hi = torch.cat([spl_t, torch.tensor([1 << 40], device="cuda")])[seg]
+    in_range = bool(((kk >= lo) & (kk < hi)).all())
+    stable = bool(((idx[1:] > idx[:-1]) | (seg[1:] != seg[:-1])).all())
+    st = prof.get("multisplit_pass")
+    avg = st["total_ms"] / st["launches"]
+    gbs = st["bytes"] / st["launches"] / (avg * 1e-3) / 1e9
+    hs = prof.get("split_hist")
+    del keys, vals, ok, ov, idx, kk, seg, lo, hi
+    torch.cuda.empty_cache()
+    return {"n": n, "buckets": nspl + 1, "ms": round((t1 - t0) * 1e3 / steps, 4),
+            "pass_ms": round(avg, 4), "pass_GB/s": round(gbs, 1), "pass_frac": round(gbs / HBM_PEAK_GBS, 4),
+            "hist_ms": round(hs["total_ms"] / hs["launches"], 4) if hs else None,
+            "verified": ok_perm and in_range and stable}
+
+
 def roofline_of(prof: dict, name: str = "onesweep_pass"):
     st = prof.get(name)
     if not st or st["launches"] == 0:
@@ -468,6 +506,8 @@ def main() -> int:
             r = run_workload(h, g, torch, dist, n, 3, 1, payload=True, profile=False)
             extra[f"kv_u32_{dist}"] = {"gkeys_s": round(r["gkeys_s"], 3), "ms": round(r["ms_per_sort"], 3),
                                        "verified": r["verified"]}
+        log("[bench] multisplit 2^28 u32+u32 into 64 buckets")
+        extra["multisplit_kv_u32_64_buckets"] = run_multisplit(h, g, torch, n)
         log(f"[bench] batched: {args.arrays} arrays, one call vs one call per array")
         extra["batched_arrays_kv_u32"] = run_batched(h, g, torch, args.arrays, 3)
         log(f"[bench] served: {args.requests} requests, serial vs served, Poisson load sweep")

@@ -117,6 +117,12 @@ class ServerStats(ctypes.Structure):
 
 
 _SIGS = {
+    "gs_multisplit": (
+        [ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p, ctypes.c_uint64,
+         ctypes.c_void_p, ctypes.c_uint32, ctypes.c_int, ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p,
+         ctypes.POINTER(ctypes.c_uint64)],
+        ctypes.c_int,
+    ),
     "gs_server_create": ([ctypes.POINTER(ctypes.c_void_p), ctypes.POINTER(ServerConfig)], ctypes.c_int),
     "gs_server_submit": (
         [ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p, ctypes.c_uint64, ctypes.c_int,
@@ -335,6 +341,18 @@ class Handle:
                 None if vals is None else _dev_ptr(vals), _dev_ptr(offsets), nseg, int(descending),
             )
         )
+
+    def multisplit(self, keys, splitters: np.ndarray, out_keys, vals=None, out_vals=None, upper: bool = False,
+                   descending: bool = False) -> np.ndarray:
+        """Stable multisplit into len(splitters) + 1 buckets; returns the bucket offsets."""
+        spl = np.ascontiguousarray(splitters, dtype=_CODE_NP[torch_dtype_code(keys)])
+        offs = np.zeros(spl.size + 2, dtype=np.uint64)
+        _check(self.lib.gs_multisplit(
+            self.h, torch_dtype_code(keys), _dev_ptr(keys), NONE if vals is None else torch_dtype_code(vals),
+            None if vals is None else _dev_ptr(vals), keys.numel(), spl.ctypes.data if spl.size else None, spl.size,
+            int(upper), int(descending), _dev_ptr(out_keys), None if out_vals is None else _dev_ptr(out_vals),
+            offs.ctypes.data_as(ctypes.POINTER(ctypes.c_uint64))))
+        return offs
 
     def merge(self, a_keys, b_keys, out_keys, a_vals=None, b_vals=None, out_vals=None, descending=False) -> None:
         vcode = NONE if a_vals is None else torch_dtype_code(a_vals)

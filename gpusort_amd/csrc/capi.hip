@@ -317,6 +317,44 @@ gs_status gs_sort_pairs_host(gs_handle* h, gs_dtype kdt, void* h_keys, gs_dtype 
   return from_hip(hipStreamSynchronize(h->stream), "synchronize");
 }
 
+gs_status gs_multisplit(gs_handle* h, gs_dtype kdt, const void* d_keys, gs_dtype vdt, const void* d_vals, uint64_t n,
+                        const void* h_splitters, uint32_t nspl, int upper, int descending, void* d_out_keys,
+                        void* d_out_vals, uint64_t* h_bucket_offsets) {
+  GS_CHECK_HANDLE(h);
+  if (!dtype_ok(kdt)) return fail(GS_INVALID_ARG, "unsupported key type %d", (int)kdt);
+  if (vdt != GS_NONE && !dtype_ok(vdt)) return fail(GS_INVALID_ARG, "unsupported value type %d", (int)vdt);
+  if (nspl > 255) return fail(GS_INVALID_ARG, "at most 255 splitters (256 buckets), got %u", nspl);
+  if (!h_bucket_offsets) return fail(GS_INVALID_ARG, "null bucket offsets");
+  if (nspl && !h_splitters) return fail(GS_INVALID_ARG, "null splitters");
+  if (n > 0 && (!d_keys || !d_out_keys || (vdt != GS_NONE && (!d_vals || !d_out_vals))))
+    return fail(GS_INVALID_ARG, "null key or value pointer");
+  const int kb = dtype_bytes(kdt), vb = val_bytes_of(vdt);
+  if (n > 0 && (d_keys == d_out_keys || (vb && d_vals == d_out_vals)))
+    return fail(GS_INVALID_ARG, "multisplit output must not alias its input");
+  for (uint32_t j = 0; j <= nspl + 1; ++j) h_bucket_offsets[j] = 0;
+  if (n == 0) return GS_OK;
+  DeviceScope ds(h->device);
+  const XformParams xf = make_xform((int)kdt, descending != 0);
+  const size_t ws_bytes = multisplit_workspace_bytes(kb, vb, n);
+  const size_t spl_off = align_up(ws_bytes, 256), base_off = spl_off + align_up((size_t)255 * 8, 256);
+  void* ws = nullptr;
+  hipError_t e = h->arena.get(Arena::kRadix, base_off + 256 * sizeof(uint64_t), h->stream, &ws);
+  if (e != hipSuccess) return from_hip(e, "multisplit workspace");
+  unsigned char* w = reinterpret_cast<unsigned char*>(ws);
+  if (nspl) e = hipMemcpyAsync(w + spl_off, h_splitters, (size_t)nspl * kb, hipMemcpyHostToDevice, h->stream);
+  uint64_t* d_base = reinterpret_cast<uint64_t*>(w + base_off);
+  if (e == hipSuccess)
+    e = multisplit(kb, vb, d_keys, d_vals, n, w + spl_off, nspl, upper ? 1 : 0, xf, d_out_keys, d_out_vals, d_base,
+                   ws, ws_bytes, h->stream, h->timer());
+  uint64_t base[256];
+  if (e == hipSuccess) e = hipMemcpyAsync(base, d_base, sizeof(base), hipMemcpyDeviceToHost, h->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
+  if (e != hipSuccess) return from_hip(e, "multisplit");
+  for (uint32_t j = 0; j <= nspl; ++j) h_bucket_offsets[j] = base[j];
+  h_bucket_offsets[nspl + 1] = n;
+  return GS_OK;
+}
+
 gs_status gs_merge(gs_handle* h, gs_dtype kdt, const void* a_k, const void* a_v, uint64_t na, const void* b_k,
                    const void* b_v, uint64_t nb, gs_dtype vdt, void* o_k, void* o_v, int descending) {
   GS_CHECK_HANDLE(h);

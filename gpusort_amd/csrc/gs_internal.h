@@ -79,6 +79,17 @@ struct SegTiles {
   uint64_t n_total;
   uint64_t n_keys;
 };
+// Stable multisplit of n keys (and payloads) into nspl + 1 <= 256 buckets by
+// nspl sorted splitters (device memory, key type): bucket = number of
+// splitters < key (upper = 1) or <= key (upper = 0), in the transform's
+// order.  Writes out_keys/out_vals bucket by bucket, input order within a
+// bucket, and the 256 exclusive bucket starts to d_bucket_base (device).
+size_t multisplit_workspace_bytes(int key_bytes, int val_bytes, uint64_t n);
+hipError_t multisplit(int key_bytes, int val_bytes, const void* keys, const void* vals, uint64_t n,
+                      const void* d_splitters, uint32_t nspl, int upper, const XformParams& xf, void* out_keys,
+                      void* out_vals, uint64_t* d_bucket_base, void* workspace, size_t workspace_bytes,
+                      hipStream_t stream, LaunchTimer* timer = nullptr);
+
 size_t radix_segments_workspace_bytes(int key_bytes, int val_bytes, uint64_t n_total, uint64_t tiles,
                                       uint32_t nslots);
 hipError_t radix_sort_segments(int key_bytes, int val_bytes, void* keys, void* vals, const SegTiles& tiles,

@@ -160,7 +160,54 @@ struct OnesweepArgs {
   const uint64_t* seg_range;  // slot -> [start, end) element offsets, 2 words
   const uint32_t* seg_base;   // [slot][pass][256] digit starts within the segment
   int passes;                 // stride of seg_base
+  // Multisplit mode only: the "digit" is the bucket of the key among nspl
+  // sorted splitters (raw keys, device memory), found in an LDS search tree.
+  const U* split_keys;
+  uint32_t nspl;
+  int upper;  // 1: bucket = #splitters < key; 0: #splitters <= key
+  int depth;  // levels of the splitter tree: 2^depth - 1 >= nspl
 };
+
+// Levels of the smallest perfect search tree that holds nspl splitters.
+inline int tree_depth(uint32_t nspl) {
+  int d = 0;
+  while (((1u << d) - 1) < nspl) ++d;
+  return d;
+}
+
+// Node of a perfect (2^depth - 1)-node search tree (heap order, root 1) that
+// holds the j-th smallest element: in-order rank r = j + 1 sits at level
+// depth - 1 - tz(r).
+__device__ inline uint32_t tree_node(uint32_t j, int depth) {
+  const uint32_t r = j + 1;
+  const uint32_t tz = __builtin_ctz(r);
+  return (1u << (depth - 1 - tz)) + (r >> (tz + 1));
+}
+
+// Bucket of an ordered key image: `depth` branch-free steps down the tree;
+// padding nodes hold the all-ones image, so the count is clamped to nspl.
+template <typename U>
+__device__ inline uint32_t tree_bucket(const U* tree, U img, int upper, uint32_t nspl, int depth) {
+  uint32_t i = 1;
+#pragma unroll
+  for (int l = 0; l < 8; ++l) {
+    if (l >= depth) break;
+    const U t = tree[i];
+    i = 2 * i + (upper ? (img > t) : (img >= t));
+  }
+  const uint32_t b = i - (1u << depth);
+  return b < nspl ? b : nspl;
+}
+
+// Loads nspl sorted splitters into the tree (image order), padding with the
+// largest image.  Every thread of the block takes part; a barrier must follow.
+template <typename U>
+__device__ inline void load_tree(U* tree, const U* spl, uint32_t nspl, int depth, const Xform<U>& xf, uint32_t tid,
+                                 uint32_t nthreads) {
+  const uint32_t nodes = (1u << depth) - 1;
+  for (uint32_t j = tid; j < nodes; j += nthreads)
+    tree[tree_node(j, depth)] = j < nspl ? to_ordered(spl[j], xf) : U(~U(0));
+}
 
 // Digit of a key: the ordered image's byte at `shift`.  IDENT is the
 // unsigned-ascending case where the image is the key itself.
@@ -170,9 +217,11 @@ __device__ inline uint32_t key_digit(U key, const Xform<U>& xf, int shift) {
   return digit_of(to_ordered(key, xf), shift);
 }
 
-// SEG: segmented mode (every pass runs; pass p reads buffer p & 1).  Idx is
-// the type of element positions: 32 bits whenever they fit.
-template <typename U, typename V, bool KV, int THREADS, int ITEMS, typename S, bool IDENT, bool SEG, typename Idx>
+// SEG: segmented mode (every pass runs; pass p reads buffer p & 1).  SPLIT:
+// multisplit mode (one pass from buffer 0 to buffer 1, digit = bucket).  Idx
+// is the type of element positions: 32 bits whenever they fit.
+template <typename U, typename V, bool KV, int THREADS, int ITEMS, typename S, bool IDENT, bool SEG, typename Idx,
+          bool SPLIT = false>
 __global__ __launch_bounds__(THREADS) void onesweep_kernel(OnesweepArgs<U, V, S> a) {
   constexpr int WAVES = THREADS / kWave;
   constexpr uint32_t TILE = THREADS * ITEMS;
@@ -192,9 +241,13 @@ __global__ __launch_bounds__(THREADS) void onesweep_kernel(OnesweepArgs<U, V, S>
   __shared__ uint32_t s_wsum[4];
   __shared__ uint32_t s_tile;
   __shared__ __attribute__((aligned(16))) unsigned char s_buf[TILE * BUF_ELEM];
+  __shared__ U s_tree[SPLIT ? 256 : 1];
+  __shared__ unsigned char s_bkt[SPLIT ? TILE : 1];  // bucket of the key in each slot
 
   uint32_t src;
-  if constexpr (SEG) {
+  if constexpr (SPLIT) {
+    src = 0;
+  } else if constexpr (SEG) {
     src = (uint32_t)a.pass & 1u;
   } else {
     src = a.ctrl->src[a.pass];
@@ -215,7 +268,13 @@ __global__ __launch_bounds__(THREADS) void onesweep_kernel(OnesweepArgs<U, V, S>
     s_wave_hist[i] = 0;
     s_match[i] = 0;
   }
+  if constexpr (SPLIT) load_tree<U>(s_tree, a.split_keys, a.nspl, a.depth, a.xf, tid, THREADS);
   lds_barrier();
+  // Digit of a key: its byte at `shift`, or its bucket in multisplit mode.
+  auto digit_of_key = [&](U key) -> uint32_t {
+    if constexpr (SPLIT) return tree_bucket<U>(s_tree, to_ordered(key, a.xf), a.upper, a.nspl, a.depth);
+    return key_digit<U, IDENT>(key, a.xf, shift);
+  };
   const uint32_t tile = s_tile;
   uint64_t tile_base, remain;
   uint32_t first = 0;       // first tile of this tile's segment
@@ -261,8 +320,20 @@ __global__ __launch_bounds__(THREADS) void onesweep_kernel(OnesweepArgs<U, V, S>
 
   // Keys past the end of the array take digit 255: they are the last keys of
   // the tile in key order, so they land on the last (TILE - valid) slots.
+  // Buckets (multisplit) are searched once and kept packed, 4 per register.
+  uint32_t bpk[SPLIT ? (ITEMS + 3) / 4 : 1];
+  if constexpr (SPLIT) {
+#pragma unroll
+    for (int q = 0; q < (ITEMS + 3) / 4; ++q) bpk[q] = 0;
+#pragma unroll
+    for (int i = 0; i < ITEMS; ++i) {
+      const uint32_t d = (full || (lofs + i * kWave) < valid) ? digit_of_key(k[i]) : 255u;
+      bpk[i / 4] |= d << (8 * (i % 4));
+    }
+  }
   auto digit_at = [&](int i) -> uint32_t {
-    const uint32_t d = key_digit<U, IDENT>(k[i], a.xf, shift);
+    if constexpr (SPLIT) return (bpk[i / 4] >> (8 * (i % 4))) & 0xFFu;
+    const uint32_t d = digit_of_key(k[i]);
     if (full) return d;
     return (lofs + i * kWave) < valid ? d : 255u;
   };
@@ -328,6 +399,7 @@ __global__ __launch_bounds__(THREADS) void onesweep_kernel(OnesweepArgs<U, V, S>
   for (int i = 0; i < ITEMS; ++i) {
     r[i] += wh[digit_at(i)];
     s_k[r[i]] = k[i];
+    if constexpr (SPLIT) s_bkt[r[i]] = (unsigned char)digit_at(i);
   }
 
   // 5. decoupled look-back, one lane per digit.  Each step reads a window of
@@ -378,7 +450,7 @@ __global__ __launch_bounds__(THREADS) void onesweep_kernel(OnesweepArgs<U, V, S>
     const uint32_t p = (uint32_t)j * THREADS + tid;
     if (p < valid) {
       const U key = s_k[p];
-      const Idx dst = s_gofs[key_digit<U, IDENT>(key, a.xf, shift)] + (Idx)p;
+      const Idx dst = s_gofs[SPLIT ? (uint32_t)s_bkt[p] : digit_of_key(key)] + (Idx)p;
       kout[dst] = key;
       if constexpr (KV) g[j] = dst;
     }
@@ -784,6 +856,161 @@ hipError_t radix_sort_segments(int kb, int vb, void* keys, void* vals, const Seg
     if (vb == 0) return radix_segments_i<uint64_t, 0>(k, vals, T, xf, ws, L, stream, timer);
     if (vb == 4) return radix_segments_i<uint64_t, 4>(k, vals, T, xf, ws, L, stream, timer);
     if (vb == 8) return radix_segments_i<uint64_t, 8>(k, vals, T, xf, ws, L, stream, timer);
+  }
+  return hipErrorInvalidValue;
+}
+
+// ------------------------------------------------------------------ multisplit
+//
+// Stable multisplit of unsorted keys (and payloads) into nspl + 1 <= 256
+// buckets: one counting read (split_hist_kernel), the scan, and one onesweep
+// pass whose digit is the key's bucket, found by log2(nspl + 1) steps down a
+// perfect splitter tree in LDS.  Output goes to separate buffers, bucket after bucket,
+// keys of a bucket in input order.
+
+template <typename U>
+__global__ __launch_bounds__(256) void split_hist_kernel(const U* __restrict__ keys, uint64_t n, Xform<U> xf,
+                                                         const U* __restrict__ spl, uint32_t nspl, int upper,
+                                                         int depth, unsigned long long* __restrict__ g_hist,
+                                                         uint4* __restrict__ zero_ptr, uint64_t zero_vec4) {
+  __shared__ uint32_t s_hist[kRadix];
+  __shared__ U s_tree[256];
+  const uint32_t tid = threadIdx.x;
+  s_hist[tid] = 0;
+  load_tree<U>(s_tree, spl, nspl, depth, xf, tid, 256);
+  const uint64_t gtid = (uint64_t)blockIdx.x * 256 + tid;
+  const uint64_t gstride = (uint64_t)gridDim.x * 256;
+  for (uint64_t i = gtid; i < zero_vec4; i += gstride) zero_ptr[i] = make_uint4(0, 0, 0, 0);
+  __syncthreads();
+  for (uint64_t i = gtid; i < n; i += gstride)
+    hist_add<U>(s_hist, tree_bucket<U>(s_tree, to_ordered(keys[i], xf), upper, nspl, depth));
+  __syncthreads();
+  if (s_hist[tid]) atomicAdd(&g_hist[tid], (unsigned long long)s_hist[tid]);
+}
+
+struct SplitLayout {
+  size_t off_ctrl, off_hist, off_base, off_status, total;
+  uint64_t tiles;
+  size_t status_word;
+};
+
+static SplitLayout split_layout(int kb, int vb, uint64_t n) {
+  SplitLayout L{};
+  L.tiles = (n + tile_keys_for(kb, vb) - 1) / tile_keys_for(kb, vb);
+  L.status_word = (n < (1ull << 30)) ? 4 : 8;
+  size_t o = 0;
+  L.off_ctrl = o;
+  o = align_up(o + sizeof(SortCtrl), 256);
+  L.off_hist = o;
+  o = align_up(o + kRadix * sizeof(unsigned long long), 256);
+  L.off_base = o;
+  o = align_up(o + kRadix * sizeof(uint64_t), 256);
+  L.off_status = o;
+  o = align_up(o + L.tiles * kRadix * L.status_word, 256);
+  L.total = o;
+  return L;
+}
+
+size_t multisplit_workspace_bytes(int key_bytes, int val_bytes, uint64_t n) {
+  return split_layout(key_bytes, val_bytes, n).total;
+}
+
+template <typename U, int VB, typename S>
+static hipError_t multisplit_t(const U* keys, const void* vals, uint64_t n, const U* d_spl, uint32_t nspl, int upper,
+                               const XformParams& xp, U* out_keys, void* out_vals, unsigned char* ws,
+                               const SplitLayout& L, uint64_t* d_bucket_base, hipStream_t stream,
+                               LaunchTimer* timer) {
+  using V = typename ValT<VB>::T;
+  using Idx = typename std::conditional<sizeof(S) == 4, uint32_t, uint64_t>::type;
+  constexpr bool KV = VB != 0;
+  constexpr int THREADS = Geometry<U, VB>::THREADS;
+  constexpr int ITEMS = Geometry<U, VB>::ITEMS;
+  const Xform<U> xf = xform_of<U>(xp);
+  SortCtrl* ctrl = reinterpret_cast<SortCtrl*>(ws + L.off_ctrl);
+  unsigned long long* hist = reinterpret_cast<unsigned long long*>(ws + L.off_hist);
+  uint64_t* base = d_bucket_base ? d_bucket_base : reinterpret_cast<uint64_t*>(ws + L.off_base);
+  S* status = reinterpret_cast<S*>(ws + L.off_status);
+  hipLaunchKernelGGL(fill_u32_kernel, dim3(1), dim3(256), 0, stream, reinterpret_cast<uint32_t*>(hist),
+                     (uint64_t)(kRadix * 2), 0u);
+  const uint64_t status_bytes = L.tiles * kRadix * sizeof(S);
+  uint64_t blocks = (n + 256 * 16 - 1) / (256 * 16);
+  if (blocks < 1) blocks = 1;
+  if (blocks > 2048) blocks = 2048;
+  if (timer) timer->begin("split_hist", n * sizeof(U) + status_bytes, stream);
+  const int depth = tree_depth(nspl);
+  hipLaunchKernelGGL(split_hist_kernel<U>, dim3((uint32_t)blocks), dim3(256), 0, stream, keys, n, xf, d_spl, nspl,
+                     upper, depth, hist, reinterpret_cast<uint4*>(status), status_bytes / 16);
+  if (timer) timer->end(stream);
+  hipLaunchKernelGGL(radix_scan_kernel, dim3(1), dim3(256), 0, stream, hist, base, ctrl, 1, n);
+  OnesweepArgs<U, V, S> a{};
+  a.keys0 = const_cast<U*>(keys);
+  a.keys1 = out_keys;
+  a.vals0 = const_cast<V*>(reinterpret_cast<const V*>(vals));
+  a.vals1 = reinterpret_cast<V*>(out_vals);
+  a.n = n;
+  a.xf = xf;
+  a.pass = 0;
+  a.shift = 0;
+  a.digit_base = base;
+  a.status = status;
+  a.ctrl = ctrl;
+  a.tiles = (uint32_t)L.tiles;
+  a.split_keys = d_spl;
+  a.nspl = nspl;
+  a.upper = upper;
+  a.depth = depth;
+  if (timer) timer->begin("multisplit_pass", 2 * n * (sizeof(U) + (KV ? sizeof(V) : 0)), stream);
+  hipLaunchKernelGGL((onesweep_kernel<U, V, KV, THREADS, ITEMS, S, false, false, Idx, true>), dim3((uint32_t)L.tiles),
+                     dim3(THREADS), 0, stream, a);
+  if (timer) timer->end(stream);
+  return hipGetLastError();
+}
+
+template <typename U, int VB>
+static hipError_t multisplit_s(const U* keys, const void* vals, uint64_t n, const void* d_spl, uint32_t nspl,
+                               int upper, const XformParams& xp, void* out_keys, void* out_vals, unsigned char* ws,
+                               const SplitLayout& L, uint64_t* d_bucket_base, hipStream_t stream,
+                               LaunchTimer* timer) {
+  const U* spl = reinterpret_cast<const U*>(d_spl);
+  U* ok = reinterpret_cast<U*>(out_keys);
+  if (L.status_word == 4)
+    return multisplit_t<U, VB, uint32_t>(keys, vals, n, spl, nspl, upper, xp, ok, out_vals, ws, L, d_bucket_base,
+                                         stream, timer);
+  return multisplit_t<U, VB, uint64_t>(keys, vals, n, spl, nspl, upper, xp, ok, out_vals, ws, L, d_bucket_base,
+                                       stream, timer);
+}
+
+hipError_t multisplit(int kb, int vb, const void* keys, const void* vals, uint64_t n, const void* d_splitters,
+                      uint32_t nspl, int upper, const XformParams& xf, void* out_keys, void* out_vals,
+                      uint64_t* d_bucket_base, void* workspace, size_t workspace_bytes, hipStream_t stream,
+                      LaunchTimer* timer) {
+  if (n == 0) return hipSuccess;
+  if (nspl > 255) return hipErrorInvalidValue;
+  const SplitLayout L = split_layout(kb, vb, n);
+  if (workspace_bytes < L.total) return hipErrorInvalidValue;
+  unsigned char* ws = reinterpret_cast<unsigned char*>(workspace);
+  if (kb == 4) {
+    const uint32_t* k = reinterpret_cast<const uint32_t*>(keys);
+    if (vb == 0)
+      return multisplit_s<uint32_t, 0>(k, vals, n, d_splitters, nspl, upper, xf, out_keys, out_vals, ws, L,
+                                       d_bucket_base, stream, timer);
+    if (vb == 4)
+      return multisplit_s<uint32_t, 4>(k, vals, n, d_splitters, nspl, upper, xf, out_keys, out_vals, ws, L,
+                                       d_bucket_base, stream, timer);
+    if (vb == 8)
+      return multisplit_s<uint32_t, 8>(k, vals, n, d_splitters, nspl, upper, xf, out_keys, out_vals, ws, L,
+                                       d_bucket_base, stream, timer);
+  } else if (kb == 8) {
+    const uint64_t* k = reinterpret_cast<const uint64_t*>(keys);
+    if (vb == 0)
+      return multisplit_s<uint64_t, 0>(k, vals, n, d_splitters, nspl, upper, xf, out_keys, out_vals, ws, L,
+                                       d_bucket_base, stream, timer);
+    if (vb == 4)
+      return multisplit_s<uint64_t, 4>(k, vals, n, d_splitters, nspl, upper, xf, out_keys, out_vals, ws, L,
+                                       d_bucket_base, stream, timer);
+    if (vb == 8)
+      return multisplit_s<uint64_t, 8>(k, vals, n, d_splitters, nspl, upper, xf, out_keys, out_vals, ws, L,
+                                       d_bucket_base, stream, timer);
   }
   return hipErrorInvalidValue;
 }

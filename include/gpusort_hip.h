@@ -190,6 +190,20 @@ typedef struct gs_kernel_stat {
 gs_status gs_profile_begin(gs_handle* h);
 gs_status gs_profile_end(gs_handle* h, gs_kernel_stat* out, uint32_t max_out, uint32_t* n_out);
 
+/* Stable multisplit of unsorted device keys (and payloads) into nspl + 1
+ * buckets, nspl <= 255, by the sorted splitters h_splitters (host, key type,
+ * sorted in the direction's order): a key's bucket is the number of
+ * splitters <= key (upper = 0: the lower_bound cut of HykSort's TransferData,
+ * include/helpers/hyksort_helper.h:563-568) or < key (upper = 1: the
+ * upper_bound cut of SampleSort's, include/helpers/samplesort_helper.h:
+ * 525-537).  On sorted input the bucket offsets equal those cuts; the kernel
+ * also takes unsorted input.  d_out_keys / d_out_vals (not aliasing the
+ * input) receive the buckets in order, keys of a bucket in input order;
+ * h_bucket_offsets receives nspl + 2 offsets (0, ..., n).  Synchronous. */
+gs_status gs_multisplit(gs_handle* h, gs_dtype kdt, const void* d_keys, gs_dtype vdt, const void* d_vals, uint64_t n,
+                        const void* h_splitters, uint32_t nspl, int upper, int descending, void* d_out_keys,
+                        void* d_out_vals, uint64_t* h_bucket_offsets);
+
 /* Host-memory form of gs_vrank_sort: the arrays are staged through the
  * handle's arena and copied back in place. */
 gs_status gs_vrank_sort_host(gs_handle* h, gs_algorithm algo, gs_dtype kdt, void* h_keys, gs_dtype vdt, void* h_vals,

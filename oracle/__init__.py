@@ -156,6 +156,18 @@ def seg_stable_sort_pairs(k: np.ndarray, v: np.ndarray, offsets: np.ndarray, des
     return kk, vv
 
 
+def multisplit(keys: np.ndarray, splitters: np.ndarray, upper: bool = False, descending: bool = False):
+    """Stable bucket partition: (permutation, offsets of the nspl + 1 buckets)."""
+    k = np.ascontiguousarray(keys)
+    spl = np.ascontiguousarray(splitters, dtype=k.dtype)
+    idx = np.empty(k.size, dtype=np.int64)
+    offs = np.empty(spl.size + 2, dtype=np.int64)
+    fn = getattr(lib(), f"orc_multisplit_{KEY_NAMES[k.dtype]}")
+    fn(_p(k), ctypes.c_int64(k.size), _p(spl), ctypes.c_int64(spl.size), ctypes.c_int(int(upper)),
+       ctypes.c_int(int(descending)), _p(idx), _p(offs))
+    return idx, offs
+
+
 def merge(a: np.ndarray, b: np.ndarray, descending: bool = False) -> np.ndarray:
     """OmpUtils::Merge restated (omp_utils.h:54-145)."""
     a = np.ascontiguousarray(a)

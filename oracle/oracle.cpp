@@ -535,6 +535,24 @@ int orc_get_threads(void) { return max_threads(); }
         std::sort(a + off[s], a + off[s + 1], std::less<K>());                                   \
     }                                                                                            \
   }                                                                                              \
+  /* Bucket partition of unsorted keys: bucket = splitters <= key (upper 0, */ \
+  /* the lower_bound cut of hyksort_helper.h:563-568) or < key (upper 1,     */ \
+  /* samplesort_helper.h:525-537); stable within a bucket.                   */ \
+  void orc_multisplit_##kn(const K* keys, int64_t n, const K* spl, int64_t nspl, int upper,      \
+                           int desc, int64_t* out_index, int64_t* offsets) {                     \
+    std::vector<int64_t> b(n), cnt(nspl + 2, 0);                                                 \
+    for (int64_t i = 0; i < n; i++) {                                                            \
+      const K* p = desc ? (upper ? std::lower_bound(spl, spl + nspl, keys[i], std::greater<K>())  \
+                                 : std::upper_bound(spl, spl + nspl, keys[i], std::greater<K>())) \
+                        : (upper ? std::lower_bound(spl, spl + nspl, keys[i], std::less<K>())     \
+                                 : std::upper_bound(spl, spl + nspl, keys[i], std::less<K>()));   \
+      b[i] = p - spl;                                                                            \
+      cnt[b[i] + 1]++;                                                                           \
+    }                                                                                            \
+    for (int64_t j = 0; j <= nspl; j++) cnt[j + 1] += cnt[j];                                    \
+    for (int64_t j = 0; j <= nspl + 1; j++) offsets[j] = cnt[j];                                 \
+    for (int64_t i = 0; i < n; i++) out_index[cnt[b[i]]++] = i;                                  \
+  }                                                                                              \
   void orc_merge_##kn(const K* a, int64_t na, const K* b, int64_t nb, K* out, int desc) {        \
     if (desc)                                                                                    \
       omp_merge(std::greater<K>(), a, a + na, b, b + nb, out);                                   \

@@ -267,3 +267,20 @@ def test_segmented_oracle_matches_lexsort(descending):
     kk, vv = oracle.seg_stable_sort_pairs(k, v, off, descending)
     assert np.array_equal(kk, k[order]) and np.array_equal(vv, v[order])
     assert np.array_equal(oracle.seg_sort(k, off, descending), k[order])
+
+
+@pytest.mark.parametrize("upper", [False, True])
+@pytest.mark.parametrize("descending", [False, True])
+def test_multisplit_oracle_matches_searchsorted(upper, descending):
+    # Bucket = searchsorted side (closed form); stable partition = stable argsort by bucket.
+    rng = np.random.default_rng(11)
+    keys = rng.integers(0, 60, 5000).astype(np.int64)
+    spl = np.sort(rng.choice(keys, 9, replace=False))
+    if descending:
+        spl = spl[::-1].copy()
+        b = np.searchsorted(-spl, -keys, side="left" if upper else "right")
+    else:
+        b = np.searchsorted(spl, keys, side="left" if upper else "right")
+    idx, offs = oracle.multisplit(keys, spl, upper, descending)
+    assert np.array_equal(idx, np.argsort(b, kind="stable"))
+    assert np.array_equal(offs, np.concatenate([[0], np.cumsum(np.bincount(b, minlength=spl.size + 1))]))
