@@ -356,6 +356,96 @@ def run_multisplit(h, g, torch, n: int, nspl: int = 63, steps: int = 5):
             "verified": ok_perm and in_range and stable}
 
 
+def run_c1(h, g, torch, with_cpu: bool):
+    """Plain single-rank sort: 2^24 uniform u32 keys from seed 1, keys only;
+    the GPU result must equal the reference CPU output (OmpUtils::MergeSort,
+    what single-rank SampleSort / HykSort run) exactly."""
+    from gpusort_amd import datagen
+
+    n = 1 << 24
+    keys = torch.empty(n, dtype=torch.uint32, device="cuda")
+    h.generate(keys, g.GEN_UNIFORM_U32, SEED)
+    k = keys.clone()
+    h.sort_keys(k)
+    copies = [keys.clone() for _ in range(10)]
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for c in copies:
+        h.sort_keys(c)
+    torch.cuda.synchronize()
+    ms = (time.perf_counter() - t0) * 1e3 / len(copies)
+    out = {"n": n, "gpu_ms": round(ms, 4), "gpu_gkeys_s": round(n / (ms * 1e-3) / 1e9, 3)}
+    if with_cpu:
+        import oracle
+
+        host_keys = datagen.uniform_u32(SEED, n)
+        same_input = bool(np.array_equal(keys.cpu().numpy(), host_keys))
+        fn = oracle.ref_merge_sort if oracle.ref_lib() is not None else oracle.merge_sort
+        t0 = time.perf_counter()
+        want = fn(host_keys)
+        cpu_s = time.perf_counter() - t0
+        out.update({"equals_reference_cpu_output": same_input and bool(np.array_equal(copies[-1].cpu().numpy(), want)),
+                    "cpu_kind": "reference" if oracle.ref_lib() is not None else "port",
+                    "cpu_s": round(cpu_s, 3), "cpu_cores": oracle.threads()})
+    del copies, keys, k
+    torch.cuda.empty_cache()
+    return out
+
+
+def run_c3(h, g, torch, with_cpu: bool, steps: int = 3):
+    """Per-rank pipeline on one GPU: 2^27 uniform u64 keys with u32 payloads
+    (original index) in 64 virtual ranks of 2^21; HykSort rounds with k = 8
+    and 32 regular samples per rank.  Checked against the CPU re-run of the
+    same rule (oracle): output, splitters, bucket sizes and final rank sizes;
+    the CPU re-run's time is the host baseline for this configuration."""
+    from gpusort_amd import datagen
+
+    p, n = 64, 1 << 27
+    off = np.arange(p + 1, dtype=np.uint64) * np.uint64(n // p)
+    keys = torch.empty(n, dtype=torch.uint64, device="cuda")
+    h.generate(keys, g.GEN_UNIFORM_U64, 3)
+    vals = torch.empty(n, dtype=torch.uint32, device="cuda")
+    h.generate(vals, g.GEN_IOTA_U32, 0)
+    k, v = keys.clone(), vals.clone()
+    h.vrank_sort(k, off, v, algorithm=g.HYKSORT, k_way=8, samples_per_rank=32)
+    copies = [(keys.clone(), vals.clone()) for _ in range(steps)]
+    torch.cuda.synchronize()
+    h.profile_begin()
+    t0 = time.perf_counter()
+    for k, v in copies:
+        rep = h.vrank_sort(k, off, v, algorithm=g.HYKSORT, k_way=8, samples_per_rank=32)
+    torch.cuda.synchronize()
+    ms = (time.perf_counter() - t0) * 1e3 / steps
+    prof = h.profile_end()
+    out = {"p": p, "k_way": 8, "samples_per_rank": 32, "n": n, "rounds": rep["rounds"],
+           "gpu_ms": round(ms, 3), "gpu_gkeys_s": round(n / (ms * 1e-3) / 1e9, 3),
+           "max_rank_error": rep["max_rank_error"],
+           "kernels_ms": {name: round(st["total_ms"] / steps, 3) for name, st in prof.items()}}
+    if with_cpu:
+        import oracle
+
+        hk = datagen.uniform_u64(3, n)
+        hv = np.arange(n, dtype=np.uint32)
+        same_input = bool(np.array_equal(keys.cpu().numpy(), hk))
+        t0 = time.perf_counter()
+        wk, wv, wrep = oracle.vrank(hk, off.astype(np.int64), hv, algorithm="hyksort", k_way=8, samples_per_rank=32)
+        cpu_s = time.perf_counter() - t0
+        gk, gv = copies[-1]
+        out["checks"] = {
+            "input_matches_host_generator": same_input,
+            "keys_and_payloads": bool(np.array_equal(gk.cpu().numpy(), wk)) and bool(np.array_equal(gv.cpu().numpy(), wv)),
+            "splitters": bool(np.array_equal(rep["splitters"], wrep["splitters"])),
+            "bucket_sizes": bool(np.array_equal(rep["bucket_sizes"].astype(np.int64), wrep["bucket_sizes"])),
+            "final_rank_sizes": bool(np.array_equal(rep["final_rank_sizes"].astype(np.int64), wrep["final_rank_sizes"])),
+        }
+        out["cpu_pipeline"] = {"value": round(n / cpu_s / 1e9, 4), "unit": "Gkeys/s", "seconds": round(cpu_s, 3),
+                               "cores": oracle.threads(), "kind": "port",
+                               "what": "the oracle's re-run of the same per-rank rounds on the host"}
+    del copies, keys, vals, k, v
+    torch.cuda.empty_cache()
+    return out
+
+
 def roofline_of(prof: dict, name: str = "onesweep_pass"):
     st = prof.get(name)
     if not st or st["launches"] == 0:
@@ -506,6 +596,10 @@ def main() -> int:
             r = run_workload(h, g, torch, dist, n, 3, 1, payload=True, profile=False)
             extra[f"kv_u32_{dist}"] = {"gkeys_s": round(r["gkeys_s"], 3), "ms": round(r["ms_per_sort"], 3),
                                        "verified": r["verified"]}
+        log("[bench] plain single-rank sort 2^24 u32 vs the reference CPU output")
+        extra["single_rank_2^24_u32"] = run_c1(h, g, torch, not args.no_cpu)
+        log("[bench] per-rank pipeline: 2^27 u64+u32, p=64, k=8 HykSort (+ CPU re-run)")
+        extra["vrank_hyksort_2^27_u64_p64_k8"] = run_c3(h, g, torch, not args.no_cpu)
         log("[bench] multisplit 2^28 u32+u32 into 64 buckets")
         extra["multisplit_kv_u32_64_buckets"] = run_multisplit(h, g, torch, n)
         log(f"[bench] batched: {args.arrays} arrays, one call vs one call per array")

@@ -7,7 +7,7 @@
 // MPI exchange is addressing (a received bucket is read where its sender
 // holds it), and all ranks of a round are processed by one launch per step:
 //
-//   local sort   every rank's run sorted (radix / LDS tile sort per rank)
+//   local sort   every rank's run sorted by one segmented sort over all ranks
 //   samples      regular samples of every rank gathered on the device and
 //                sorted per group (tile sort / radix sort)
 //   splitters    picked from the sorted samples by the rule below
@@ -77,6 +77,32 @@ gs_status sort_segment(Pipe& P, char* k, char* v, uint64_t len, bool lex_values)
     return sort_device(P.h, P.kdt, k, P.vdt, v, len, P.desc);
   }
   return sort_device(P.h, P.kdt, k, GS_NONE, nullptr, len, P.desc);
+}
+
+// Sorts every run [off[r], off[r+1]) of the current buffer pair in place with
+// the segmented sort (one launch sequence for all runs), lexicographically
+// (payload first, then stable by key) when lex_values is set.
+gs_status sort_runs(Pipe& P, char* k, char* v, const std::vector<uint64_t>& off, bool lex_values) {
+  const uint32_t nseg = (uint32_t)off.size() - 1;
+  std::vector<unsigned char> meta;
+  std::vector<uint32_t> huge;
+  auto pass = [&](gs_dtype kdt, char* keys, gs_dtype vdt, char* vals, int kb, int vb) -> gs_status {
+    gs_status st = sort_segments_async(P.h, kdt, keys, vdt, vals, off.data(), nseg, P.desc, &meta, &huge);
+    if (st != GS_OK) return st;
+    for (uint32_t r : huge) {
+      st = sort_device(P.h, kdt, keys + off[r] * kb, vdt, vals ? vals + off[r] * vb : nullptr, off[r + 1] - off[r],
+                       P.desc);
+      if (st != GS_OK) return st;
+    }
+    // `meta` is the source of an upload queued above.
+    return from_hip(hipStreamSynchronize(P.h->stream), "local sort");
+  };
+  if (!P.kv) return pass(P.kdt, k, GS_NONE, nullptr, P.kb, 0);
+  if (lex_values) {
+    gs_status st = pass(P.vdt, v, P.kdt, k, P.vb, P.kb);
+    if (st != GS_OK) return st;
+  }
+  return pass(P.kdt, k, P.vdt, v, P.kb, P.vb);
 }
 
 // Device aux area: a list of (host source, bytes) blocks uploaded together.
@@ -613,9 +639,9 @@ extern "C" gs_status gs_vrank_sort(gs_handle* h, gs_algorithm algo, gs_dtype kdt
     if (e != hipSuccess) return from_hip(e, "payload order check");
     lex_values = flag != 0;
   }
-  for (uint32_t r = 0; r < p; ++r) {
-    gs_status st = sort_segment(P, P.kbuf[0] + h_rank_off[r] * P.kb, P.kv ? P.vbuf[0] + h_rank_off[r] * P.vb : nullptr,
-                                sizes[r], lex_values);
+  {
+    gs_status st = sort_runs(P, P.kbuf[0], P.kv ? P.vbuf[0] : nullptr,
+                             std::vector<uint64_t>(h_rank_off, h_rank_off + p + 1), lex_values);
     if (st != GS_OK) return st;
   }
 
