@@ -138,6 +138,22 @@ hipError_t merge_tiles(int key_bytes, int val_bytes, const void* keys_a, const v
                        const uint32_t* d_tile_job, uint32_t ntiles, const XformParams& xf, uint64_t* d_split,
                        hipStream_t stream, const XformParams* lex_vxf = nullptr);
 
+// One merge-split of SampleSort's small-input bitonic network: rank list a
+// (own) and b (partner), both sorted, in one device buffer.
+struct NetCut {
+  uint64_t a_off, a_len, b_off, b_len;
+  uint32_t keep_low, pad;
+};
+// Kept length, and the merge-path split (a_split + b_split = the kept
+// prefix's length, or the length of what precedes the kept suffix).
+struct NetCutResult {
+  uint64_t keep, a_split, b_split;
+  uint32_t prefix, pad;
+};
+hipError_t net_cuts(int key_bytes, int val_bytes, const void* keys, const void* vals, const NetCut* d_ops,
+                    uint32_t nops, const XformParams& xf, const XformParams* lex_vxf, int increasing,
+                    NetCutResult* d_out, hipStream_t stream);
+
 // ---------------------------------------------------------------- splitting
 // Bucket boundaries of sorted runs.  Run r is keys[run_off[r] .. run_off[r+1])
 // and is cut by splitters[spl_first[r] .. spl_first[r] + nspl).  Output:

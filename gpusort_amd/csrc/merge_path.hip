@@ -210,6 +210,132 @@ hipError_t merge_tiles(int kb, int vb, const void* ska, const void* skb, const v
 
 // Host-side tiling of a job list: fills the device job records and the
 // tile -> job table.  Returns the number of tiles.
+// ------------------------------------------------------ bitonic merge-split cuts
+//
+// One merge-split of SampleSort's small-input bitonic network keeps a prefix
+// or a suffix of merge(a, b) (MergeLists, include/helpers/samplesort_helper.h:
+// 305-387).  The reference walks the merged list with loop conditions built
+// from low = max of the first elements, high = min of the last ones (in the
+// natural order; last/first swap for a decreasing order) and half = |a+b|/2;
+// every condition holds on a prefix (or suffix) of the sorted merge, so the
+// kept length is a min/max of element counts, found here with binary searches
+// on a and b, in the sort order of the ordered images (which is the natural
+// order, reversed for a decreasing sort):
+//   increasing, keep low   L = min(max(#(< low), half), #(<= high))     prefix
+//   increasing, keep high  L = max(min(n - half, #(>= low)), #(> high)) suffix
+//   decreasing, keep low   L = max(min(#(>= high'), n - half), #(> low')) suffix
+//   decreasing, keep high  L = max(min(half, #(<= low')), #(< high'))   prefix
+// with low' = min of the last elements and high' = max of the first ones in
+// image order.  The merge-path split of the kept part is returned with it.
+
+template <typename E, typename Load>
+__device__ inline uint64_t count_less(Load f, uint64_t n, const E& x) {  // #(f(i) < x)
+  uint64_t lo = 0, hi = n;
+  while (lo < hi) {
+    const uint64_t mid = (lo + hi) >> 1;
+    if (x <= f(mid))
+      hi = mid;
+    else
+      lo = mid + 1;
+  }
+  return lo;
+}
+
+template <typename E, typename Load>
+__device__ inline uint64_t count_le(Load f, uint64_t n, const E& x) {  // #(f(i) <= x)
+  uint64_t lo = 0, hi = n;
+  while (lo < hi) {
+    const uint64_t mid = (lo + hi) >> 1;
+    if (f(mid) <= x)
+      lo = mid + 1;
+    else
+      hi = mid;
+  }
+  return lo;
+}
+
+template <typename U, typename V, bool LEX>
+__global__ __launch_bounds__(64) void net_cut_kernel(const U* __restrict__ keys, const V* __restrict__ vals,
+                                                     const NetCut* __restrict__ ops, uint32_t nops, Xform<U> xf,
+                                                     Xform<V> vxf, int increasing, NetCutResult* __restrict__ out) {
+  const uint32_t t = blockIdx.x * 64 + threadIdx.x;
+  if (t >= nops) return;
+  const NetCut op = ops[t];
+  using E = MKey<U, V, LEX>;
+  const U* A = keys + op.a_off;
+  const U* B = keys + op.b_off;
+  const V* AV = vals + op.a_off;
+  const V* BV = vals + op.b_off;
+  auto ea = [&](uint64_t i) { return E{to_ordered(A[i], xf), LEX ? to_ordered(AV[i], vxf) : V(0)}; };
+  auto eb = [&](uint64_t i) { return E{to_ordered(B[i], xf), LEX ? to_ordered(BV[i], vxf) : V(0)}; };
+  const uint64_t na = op.a_len, nb = op.b_len, n = na + nb, half = n / 2;
+  auto cnt_less = [&](const E& x) { return count_less<E>(ea, na, x) + count_less<E>(eb, nb, x); };
+  auto cnt_le = [&](const E& x) { return count_le<E>(ea, na, x) + count_le<E>(eb, nb, x); };
+  auto emax = [](const E& x, const E& y) { return x <= y ? y : x; };
+  auto emin = [](const E& x, const E& y) { return x <= y ? x : y; };
+  const E a0 = ea(0), b0 = eb(0), al = ea(na - 1), bl = eb(nb - 1);
+  uint64_t L;
+  bool prefix;
+  if (increasing) {
+    const E low = emax(a0, b0), high = emin(al, bl);
+    if (op.keep_low) {
+      const uint64_t c = cnt_less(low);
+      L = (c > half ? c : half);
+      const uint64_t h = cnt_le(high);
+      L = L < h ? L : h;
+      prefix = true;
+    } else {
+      const uint64_t ge = n - cnt_less(low), gt = n - cnt_le(high);
+      L = (n - half) < ge ? (n - half) : ge;
+      L = L > gt ? L : gt;
+      prefix = false;
+    }
+  } else {
+    const E low = emin(al, bl), high = emax(a0, b0);
+    if (op.keep_low) {
+      const uint64_t ge = n - cnt_less(high), gt = n - cnt_le(low);
+      L = ge < (n - half) ? ge : (n - half);
+      L = L > gt ? L : gt;
+      prefix = false;
+    } else {
+      const uint64_t le = cnt_le(low), lt = cnt_less(high);
+      L = half < le ? half : le;
+      L = L > lt ? L : lt;
+      prefix = true;
+    }
+  }
+  const uint64_t diag = prefix ? L : n - L;
+  const uint64_t ai = merge_path_search<E>(diag, na, nb, ea, eb);
+  out[t] = NetCutResult{L, ai, diag - ai, prefix ? 1u : 0u, 0u};
+}
+
+hipError_t net_cuts(int kb, int vb, const void* keys, const void* vals, const NetCut* d_ops, uint32_t nops,
+                    const XformParams& xf, const XformParams* lex_vxf, int increasing, NetCutResult* d_out,
+                    hipStream_t stream) {
+  if (nops == 0) return hipSuccess;
+  const dim3 grid((nops + 63) / 64), block(64);
+  const bool lex = lex_vxf != nullptr && vb != 0;
+  const XformParams vz{0, 0, 0};
+  const XformParams& vp = lex ? *lex_vxf : vz;
+#define GS_CUT(UT, VT, LEXV)                                                                                     \
+  hipLaunchKernelGGL((net_cut_kernel<UT, VT, LEXV>), grid, block, 0, stream, reinterpret_cast<const UT*>(keys),  \
+                     reinterpret_cast<const VT*>(vals), d_ops, nops,                                              \
+                     Xform<UT>{(UT)xf.float_mask, (UT)xf.sign_flip, (UT)xf.desc_mask},                           \
+                     Xform<VT>{(VT)vp.float_mask, (VT)vp.sign_flip, (VT)vp.desc_mask}, increasing, d_out);       \
+  return hipGetLastError()
+  if (kb == 4) {
+    if (!lex) { GS_CUT(uint32_t, uint32_t, false); }
+    if (vb == 4) { GS_CUT(uint32_t, uint32_t, true); }
+    if (vb == 8) { GS_CUT(uint32_t, uint64_t, true); }
+  } else if (kb == 8) {
+    if (!lex) { GS_CUT(uint64_t, uint32_t, false); }
+    if (vb == 4) { GS_CUT(uint64_t, uint32_t, true); }
+    if (vb == 8) { GS_CUT(uint64_t, uint64_t, true); }
+  }
+#undef GS_CUT
+  return hipErrorInvalidValue;
+}
+
 uint32_t merge_plan(const MergeJob* jobs, uint32_t njobs, MergeJobDev* out_jobs, std::vector<uint32_t>* tile_job) {
   const uint32_t T = merge_tile_elems();
   tile_job->clear();

@@ -198,6 +198,33 @@ struct Run {
 // buffer, in list order, ties to the earlier run) into one run placed at
 // dest_off[d] of the result.  Pairwise merge-path levels ping-pong between
 // the two buffers; the result buffer becomes current.
+// Runs 2-way merge jobs from the current buffer into the other one (one
+// launch for all jobs) and makes that buffer current.
+gs_status run_merge_jobs(Pipe& P, const std::vector<MergeJob>& jobs) {
+  std::vector<MergeJobDev> dj(jobs.size());
+  std::vector<uint32_t> tile_job;
+  const uint32_t ntiles = merge_plan(jobs.data(), (uint32_t)jobs.size(), dj.data(), &tile_job);
+  if (ntiles) {
+    AuxLayout L;
+    const size_t o_j = L.add(dj.data(), dj.size() * sizeof(MergeJobDev));
+    const size_t o_t = L.add(tile_job.data(), tile_job.size() * 4);
+    const size_t o_s = L.add(nullptr, (size_t)ntiles * 8);
+    char* base = nullptr;
+    gs_status st = upload(P, L, &base);
+    if (st != GS_OK) return st;
+    const int src = P.cur, dst = P.cur ^ 1;
+    hipError_t e = merge_tiles(P.kb, P.kv ? P.vb : 0, P.kbuf[src], P.kbuf[src], P.kv ? P.vbuf[src] : nullptr,
+                               P.kv ? P.vbuf[src] : nullptr, P.kbuf[dst], P.kv ? P.vbuf[dst] : nullptr,
+                               reinterpret_cast<const MergeJobDev*>(base + o_j),
+                               reinterpret_cast<const uint32_t*>(base + o_t), ntiles, P.kx,
+                               reinterpret_cast<uint64_t*>(base + o_s), P.h->stream, P.kv ? &P.vx : nullptr);
+    if (e == hipSuccess) e = hipStreamSynchronize(P.h->stream);
+    if (e != hipSuccess) return from_hip(e, "merge level");
+  }
+  P.cur ^= 1;
+  return GS_OK;
+}
+
 gs_status merge_lists(Pipe& P, std::vector<std::vector<Run>> lists, const std::vector<uint64_t>& dest_off) {
   for (;;) {
     bool any = false;
@@ -221,27 +248,8 @@ gs_status merge_lists(Pipe& P, std::vector<std::vector<Run>> lists, const std::v
         pos += a.len + b.len;
       }
     }
-    std::vector<MergeJobDev> dj(jobs.size());
-    std::vector<uint32_t> tile_job;
-    const uint32_t ntiles = merge_plan(jobs.data(), (uint32_t)jobs.size(), dj.data(), &tile_job);
-    if (ntiles) {
-      AuxLayout L;
-      const size_t o_j = L.add(dj.data(), dj.size() * sizeof(MergeJobDev));
-      const size_t o_t = L.add(tile_job.data(), tile_job.size() * 4);
-      const size_t o_s = L.add(nullptr, (size_t)ntiles * 8);
-      char* base = nullptr;
-      gs_status st = upload(P, L, &base);
-      if (st != GS_OK) return st;
-      const int src = P.cur, dst = P.cur ^ 1;
-      hipError_t e = merge_tiles(P.kb, P.kv ? P.vb : 0, P.kbuf[src], P.kbuf[src], P.kv ? P.vbuf[src] : nullptr,
-                                 P.kv ? P.vbuf[src] : nullptr, P.kbuf[dst], P.kv ? P.vbuf[dst] : nullptr,
-                                 reinterpret_cast<const MergeJobDev*>(base + o_j),
-                                 reinterpret_cast<const uint32_t*>(base + o_t), ntiles, P.kx,
-                                 reinterpret_cast<uint64_t*>(base + o_s), P.h->stream, P.kv ? &P.vx : nullptr);
-      if (e == hipSuccess) e = hipStreamSynchronize(P.h->stream);
-      if (e != hipSuccess) return from_hip(e, "merge level");
-    }
-    P.cur ^= 1;
+    gs_status st = run_merge_jobs(P, jobs);
+    if (st != GS_OK) return st;
     lists.swap(next);
   }
   return GS_OK;
@@ -400,6 +408,139 @@ gs_status hyksort_rounds(Pipe& P, std::vector<uint64_t>& sizes, uint32_t k_way, 
     n_p = blk;
     (*rounds)++;
   }
+  return GS_OK;
+}
+
+// ---------------------------------------------- SampleSort below 5 p^2 keys
+//
+// The reference sorts small inputs with a bitonic merge-split network across
+// the ranks (samplesort_helper.h:429-452; BitonicSort :192-235,
+// BitonicSortBinary :236-258, BitonicSortBinaryHelper :260-283, BitonicMerge
+// :170-190): partners swap lists and each keeps a prefix or suffix of the
+// merge (MergeLists :305-387), so rank sizes follow the data.  The network
+// is generated here as the reference's communicator splits produce it
+// (ParUtilsHelper::SplitCommBinary flips the upper group's rank order,
+// SplitCommBinaryNoFlip does not; src/par_utils_helper.cc:237-329); every
+// round is one cut kernel (kept lengths and merge-path splits, read back)
+// and one batched merge of the kept parts.
+
+struct NetOp {
+  uint32_t rank, partner;
+  bool keep_low;
+};
+
+uint32_t prev_pow2(uint32_t v) {  // GetPrevHighestPowerOfTwo (include/common.h:185-194)
+  v--;
+  v |= v >> 1;
+  v |= v >> 2;
+  v |= v >> 4;
+  v |= v >> 8;
+  v |= v >> 16;
+  v++;
+  return v >> 1;
+}
+
+void net_binary(const std::vector<uint32_t>& comm, bool inc, std::vector<std::vector<NetOp>>* rounds) {
+  const uint32_t npes = (uint32_t)comm.size();
+  uint32_t and_bit = 2;
+  for (uint32_t pss = 2; pss <= npes; pss *= 2, and_bit <<= 1) {
+    uint32_t dim = 0;
+    for (uint32_t x = pss; x > 1; x >>= 1) ++dim;
+    uint32_t eor_bit = 1u << (dim - 1);
+    for (uint32_t stage = 0; stage < dim; ++stage, eor_bit >>= 1) {
+      std::vector<NetOp> ops;
+      for (uint32_t r = 0; r < npes; ++r) {
+        const bool incr = (r & and_bit) == 0 ? inc : !inc;
+        const uint32_t partner = r ^ eor_bit;
+        ops.push_back(NetOp{comm[r], comm[partner], incr ? r < partner : r > partner});
+      }
+      rounds->push_back(ops);
+    }
+  }
+}
+
+void net_bitonic(const std::vector<uint32_t>& comm, bool inc, std::vector<std::vector<NetOp>>* rounds) {
+  const uint32_t npes = (uint32_t)comm.size();
+  if (npes == 1) return;
+  if (is_pow2(npes)) {
+    net_binary(comm, inc, rounds);
+    return;
+  }
+  const uint32_t splitter = prev_pow2(npes);
+  const std::vector<uint32_t> A(comm.begin(), comm.begin() + splitter);
+  std::vector<uint32_t> b_flip;
+  for (uint32_t j = 0; j < npes - splitter; ++j) b_flip.push_back(comm[npes - 1 - j]);
+  net_binary(A, inc, rounds);
+  net_bitonic(b_flip, inc, rounds);
+  const uint32_t num_left = prev_pow2(npes), num_right = npes - num_left;
+  std::vector<NetOp> ops;
+  for (uint32_t r = num_left - num_right; r < num_left; ++r) ops.push_back(NetOp{comm[r], comm[r + num_right], inc});
+  for (uint32_t r = num_left; r < npes; ++r) ops.push_back(NetOp{comm[r], comm[r - num_right], !inc});
+  rounds->push_back(ops);
+  net_binary(A, inc, rounds);
+  net_bitonic(std::vector<uint32_t>(comm.begin() + splitter, comm.end()), inc, rounds);
+}
+
+gs_status samplesort_small(Pipe& P, std::vector<uint64_t>& sizes, uint32_t* rounds) {
+  const uint32_t p = (uint32_t)sizes.size();
+  std::vector<uint32_t> comm(p);
+  for (uint32_t r = 0; r < p; ++r) comm[r] = r;
+  std::vector<std::vector<NetOp>> net;
+  net_bitonic(comm, !P.desc, &net);
+  for (const std::vector<NetOp>& round : net) {
+    std::vector<uint64_t> off(p + 1, 0);
+    for (uint32_t r = 0; r < p; ++r) off[r + 1] = off[r] + sizes[r];
+    const uint32_t nops = (uint32_t)round.size();
+    std::vector<NetCut> ops(nops);
+    for (uint32_t i = 0; i < nops; ++i) {
+      const NetOp& o = round[i];
+      ops[i] = NetCut{off[o.rank], sizes[o.rank], off[o.partner], sizes[o.partner], o.keep_low ? 1u : 0u, 0u};
+    }
+    AuxLayout L;
+    const size_t o_ops = L.add(ops.data(), nops * sizeof(NetCut));
+    const size_t o_out = L.add(nullptr, nops * sizeof(NetCutResult));
+    char* base = nullptr;
+    gs_status st = upload(P, L, &base);
+    if (st != GS_OK) return st;
+    std::vector<NetCutResult> res(nops);
+    hipError_t e = net_cuts(P.kb, P.kv ? P.vb : 0, P.kbuf[P.cur], P.kv ? P.vbuf[P.cur] : nullptr,
+                            reinterpret_cast<const NetCut*>(base + o_ops), nops, P.kx, P.kv ? &P.vx : nullptr,
+                            P.desc ? 0 : 1, reinterpret_cast<NetCutResult*>(base + o_out), P.h->stream);
+    if (e == hipSuccess)
+      e = hipMemcpyAsync(res.data(), base + o_out, nops * sizeof(NetCutResult), hipMemcpyDeviceToHost, P.h->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(P.h->stream);
+    if (e != hipSuccess) return from_hip(e, "bitonic cuts");
+    std::vector<uint64_t> ns = sizes;
+    std::vector<bool> busy(p, false);
+    for (uint32_t i = 0; i < nops; ++i) {
+      const NetOp& o = round[i];
+      const NetCutResult& c = res[i];
+      const uint64_t na = sizes[o.rank], nb = sizes[o.partner];
+      const uint64_t taken = c.a_split + c.b_split;
+      if (c.a_split > na || c.b_split > nb || c.keep == 0 || (c.prefix ? taken != c.keep : taken + c.keep != na + nb))
+        return fail(GS_DEVICE_ERROR, "inconsistent bitonic cut at rank %u", o.rank);
+      ns[o.rank] = c.keep;
+      busy[o.rank] = true;
+    }
+    std::vector<uint64_t> noff(p + 1, 0);
+    for (uint32_t r = 0; r < p; ++r) noff[r + 1] = noff[r] + ns[r];
+    std::vector<MergeJob> jobs;
+    for (uint32_t i = 0; i < nops; ++i) {
+      const NetOp& o = round[i];
+      const NetCutResult& c = res[i];
+      if (c.prefix)
+        jobs.push_back(MergeJob{off[o.rank], c.a_split, off[o.partner], c.b_split, noff[o.rank]});
+      else
+        jobs.push_back(MergeJob{off[o.rank] + c.a_split, sizes[o.rank] - c.a_split, off[o.partner] + c.b_split,
+                                sizes[o.partner] - c.b_split, noff[o.rank]});
+    }
+    for (uint32_t r = 0; r < p; ++r)
+      if (!busy[r]) jobs.push_back(MergeJob{off[r], sizes[r], off[r], 0, noff[r]});
+    st = run_merge_jobs(P, jobs);
+    if (st != GS_OK) return st;
+    sizes.swap(ns);
+  }
+  *rounds = (uint32_t)net.size();
   return GS_OK;
 }
 
@@ -591,8 +732,7 @@ extern "C" gs_status gs_vrank_sort(gs_handle* h, gs_algorithm algo, gs_dtype kdt
   // hyperquicksort_common_vector.inl:30-41).
   if (algo == GS_HYKSORT && (!is_pow2(p) || !is_pow2(k_way))) algo = GS_SAMPLESORT;
   if (algo == GS_HYPERQUICKSORT && !is_pow2(p)) algo = GS_SAMPLESORT;
-  if (algo == GS_SAMPLESORT && p > 1 && n < 5ull * p * p)
-    return fail(GS_UNSUPPORTED, "SampleSort with N < 5p^2 takes the distributed bitonic path, not built");
+  const bool small_samplesort = algo == GS_SAMPLESORT && p > 1 && n < 5ull * p * p;
 
   DeviceScope ds(h->device);
   Pipe P;
@@ -649,14 +789,20 @@ extern "C" gs_status gs_vrank_sort(gs_handle* h, gs_algorithm algo, gs_dtype kdt
   double max_err = 0.0;
   gs_status st = GS_OK;
   if (p > 1) {
-    if (algo == GS_HYKSORT)
-      st = hyksort_rounds(P, sizes, k_way, samples_per_rank, rep, &max_err, &rounds);
-    else if (algo == GS_SAMPLESORT)
-      st = samplesort_round(P, sizes, rep, &rounds);
-    else
-      st = hyperquicksort_rounds(P, sizes, samples_per_rank, rep, &rounds);
-    if (st != GS_OK) return st;
-    keep_no_empty(sizes);
+    if (small_samplesort) {
+      // No rebalance on this path: no rank ever runs empty (samplesort_helper.h:429-452).
+      st = samplesort_small(P, sizes, &rounds);
+      if (st != GS_OK) return st;
+    } else {
+      if (algo == GS_HYKSORT)
+        st = hyksort_rounds(P, sizes, k_way, samples_per_rank, rep, &max_err, &rounds);
+      else if (algo == GS_SAMPLESORT)
+        st = samplesort_round(P, sizes, rep, &rounds);
+      else
+        st = hyperquicksort_rounds(P, sizes, samples_per_rank, rep, &rounds);
+      if (st != GS_OK) return st;
+      keep_no_empty(sizes);
+    }
   }
   if (P.cur != 0) {
     e = copy_bytes(P.kbuf[0], P.kbuf[1], n * (size_t)P.kb, h->stream);

@@ -168,6 +168,18 @@ def multisplit(keys: np.ndarray, splitters: np.ndarray, upper: bool = False, des
     return idx, offs
 
 
+def merge_lists(a: np.ndarray, b: np.ndarray, keep_low: bool, increasing: bool) -> np.ndarray:
+    """MergeLists of SampleSort's small-input bitonic network (int64 lists)."""
+    a = np.ascontiguousarray(a, dtype=np.int64)
+    b = np.ascontiguousarray(b, dtype=np.int64)
+    out = np.empty(a.size + b.size, dtype=np.int64)
+    fn = lib().orc_merge_lists_i64
+    fn.restype = ctypes.c_int64
+    n = fn(_p(a), ctypes.c_int64(a.size), _p(b), ctypes.c_int64(b.size), ctypes.c_int(int(keep_low)),
+           ctypes.c_int(int(increasing)), _p(out))
+    return out[:n].copy()
+
+
 def merge(a: np.ndarray, b: np.ndarray, descending: bool = False) -> np.ndarray:
     """OmpUtils::Merge restated (omp_utils.h:54-145)."""
     a = np.ascontiguousarray(a)

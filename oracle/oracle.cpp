@@ -35,6 +35,7 @@
 #include <cmath>
 #include <cstdlib>
 #include <functional>
+#include <tuple>
 #include <utility>
 #include <vector>
 
@@ -366,6 +367,119 @@ static void vrank_samplesort(std::vector<std::vector<T>>& R, Comp comp, bool inc
   rep.rounds = 1;
 }
 
+// SampleSort's small-input path (samplesort_helper.h:429-452): below 5 p^2
+// keys in total the ranks run a bitonic merge-split network instead of
+// sampling.  Restated from BitonicSort (:192-235), BitonicSortBinary
+// (:236-258), BitonicSortBinaryHelper (:260-283), BitonicMerge (:170-190),
+// Merge (:285-303) and MergeLists (:305-387); communicator splits as
+// ParUtilsHelper::SplitCommBinary / SplitCommBinaryNoFlip
+// (src/par_utils_helper.cc:237-329).  `comm` lists the global ranks of a
+// communicator in its rank order.  Every round of merge-splits reads the
+// lists as they were before the round, as the pairwise exchanges do.
+static int prev_pow2(unsigned v) {
+  v--;
+  v |= v >> 1;
+  v |= v >> 2;
+  v |= v >> 4;
+  v |= v >> 8;
+  v |= v >> 16;
+  v++;
+  return (int)(v >> 1);
+}
+
+// MergeLists: list_a (own) merged with list_b (the partner's); keeps the
+// low or high part by the reference's loop conditions, element operators
+// and std::max / std::min as written there.
+template <typename T>
+static std::vector<T> merge_lists_ref(const std::vector<T>& list_a, const std::vector<T>& list_b, bool keep_low,
+                                      bool inc) {
+  const T low = inc ? std::max(list_a[0], list_b[0]) : std::max(list_a.back(), list_b.back());
+  const T high = inc ? std::min(list_a.back(), list_b.back()) : std::min(list_a[0], list_b[0]);
+  const int64_t n = (int64_t)(list_a.size() + list_b.size());
+  std::vector<T> s(n);
+  size_t i1 = 0, i2 = 0;
+  for (int64_t i = 0; i < n; i++) {
+    const bool take_a = i1 < list_a.size() &&
+                        (i2 >= list_b.size() || (inc ? list_a[i1] <= list_b[i2] : list_a[i1] > list_b[i2]));
+    s[i] = take_a ? list_a[i1++] : list_b[i2++];
+  }
+  const int64_t half = n / 2;
+  if (keep_low) {
+    if (inc) {
+      int64_t ii = 0;
+      while (ii < n && (s[ii] < low || ii < half) && s[ii] <= high) ii++;
+      return std::vector<T>(s.begin(), s.begin() + ii);
+    }
+    int64_t ii = n - 1;
+    while (ii >= 0 && ((s[ii] <= high && ii >= half) || s[ii] < low)) ii--;
+    return std::vector<T>(s.begin() + ii + 1, s.end());
+  }
+  if (inc) {
+    int64_t ii = n - 1;
+    while (ii >= 0 && ((ii >= half && s[ii] >= low) || s[ii] > high)) ii--;
+    return std::vector<T>(s.begin() + ii + 1, s.end());
+  }
+  int64_t ii = 0;
+  while (ii < n && ((ii < half && s[ii] >= low) || s[ii] > high)) ii++;
+  return std::vector<T>(s.begin(), s.begin() + ii);
+}
+
+template <typename T>
+struct BitonicNet {
+  std::vector<std::vector<T>>& R;
+  bool inc;
+  int rounds = 0;
+
+  // One round: (rank, partner, keep_low) for every participating rank.
+  void round(const std::vector<std::tuple<int, int, bool>>& ops) {
+    std::vector<std::vector<T>> next;
+    for (const auto& op : ops) next.push_back(merge_lists_ref(R[std::get<0>(op)], R[std::get<1>(op)], std::get<2>(op), inc));
+    for (size_t i = 0; i < ops.size(); i++) R[std::get<0>(ops[i])] = std::move(next[i]);
+    rounds++;
+  }
+  void sort_binary(const std::vector<int>& comm) {
+    const int npes = (int)comm.size();
+    unsigned and_bit = 2;
+    for (int pss = 2; pss <= npes; pss *= 2, and_bit <<= 1) {
+      int dim = 0;
+      for (int x = pss; x > 1; x >>= 1) dim++;
+      int eor_bit = 1 << (dim - 1);
+      for (int stage = 0; stage < dim; stage++) {
+        std::vector<std::tuple<int, int, bool>> ops;
+        for (int r = 0; r < npes; r++) {
+          const bool incr = ((unsigned)r & and_bit) == 0 ? inc : !inc;
+          const int partner = r ^ eor_bit;
+          const bool cond = incr ? r < partner : r > partner;
+          ops.emplace_back(comm[r], comm[partner], cond);
+        }
+        round(ops);
+        eor_bit >>= 1;
+      }
+    }
+  }
+  void sort(const std::vector<int>& comm) {
+    const int npes = (int)comm.size();
+    if (npes == 1) return;
+    if ((npes & (npes - 1)) == 0) {
+      sort_binary(comm);
+      return;
+    }
+    const int splitter = prev_pow2((unsigned)npes);
+    std::vector<int> A(comm.begin(), comm.begin() + splitter), Bflip;
+    for (int j = 0; j < npes - splitter; j++) Bflip.push_back(comm[npes - 1 - j]);
+    sort_binary(A);
+    sort(Bflip);
+    const int num_left = prev_pow2((unsigned)npes), num_right = npes - num_left;
+    std::vector<std::tuple<int, int, bool>> ops;
+    for (int r = num_left - num_right; r < num_left; r++) ops.emplace_back(comm[r], comm[r + num_right], inc);
+    for (int r = num_left; r < npes; r++) ops.emplace_back(comm[r], comm[r - num_right], !inc);
+    round(ops);
+    std::vector<int> B(comm.begin() + splitter, comm.end());
+    sort_binary(A);
+    sort(B);
+  }
+};
+
 // HyperQuickSort (hyperquicksort_helper.h:261-346) with the pivot chosen among
 // s regular samples per rank (gathered in rank order): the first sample whose
 // global lower_bound rank is closest to N_group/2 (:251-256).  Ranks
@@ -473,6 +587,12 @@ static int run_vrank(int algo, K* keys, V* vals, const int64_t* rank_off, int p,
     vrank_hyksort(R, k_way, s, lcomp, rep, spl, total);
   } else if (algo == 2 && pow2p) {
     vrank_hyperquicksort(R, s, lcomp, rep, spl);
+  } else if (p > 1 && total < 5ll * p * p) {
+    BitonicNet<T> net{R, increasing};
+    std::vector<int> comm(p);
+    for (int r = 0; r < p; r++) comm[r] = r;
+    net.sort(comm);
+    rep.rounds = net.rounds;
   } else {
     vrank_samplesort(R, lcomp, increasing, rep, spl);
   }
@@ -636,6 +756,16 @@ int orc_get_threads(void) { return max_threads(); }
 ORC_KEY_TYPES(DEF_KEYS)
 
 void orc_scan_i64(int64_t cnt, const int64_t* a, int64_t* b) { omp_scan(cnt, a, b); }
+
+// MergeLists of the small-input bitonic network on int64 lists (a own, b
+// partner's); returns the kept list's length, written to out.
+int64_t orc_merge_lists_i64(const int64_t* a, int64_t na, const int64_t* b, int64_t nb, int keep_low, int inc,
+                            int64_t* out) {
+  const std::vector<int64_t> va(a, a + na), vb(b, b + nb);
+  const std::vector<int64_t> r = merge_lists_ref(va, vb, keep_low != 0, inc != 0);
+  std::copy(r.begin(), r.end(), out);
+  return (int64_t)r.size();
+}
 
 void orc_keep_no_empty(int64_t* sizes, int p) {
   std::vector<int64_t> s(sizes, sizes + p);

@@ -213,6 +213,22 @@ void CaseSampleSortPairs() {
   Report("samplesort_f64_pairs_p8", ok);
 }
 
+void CaseSampleSortSmallInput() {
+  // Fewer than 5 p^2 keys: the bitonic merge-split network, rank sizes set by the data.
+  const size_t p = 8, n = 200;
+  std::vector<uint32_t> k = Keys<uint32_t>(23, n, 50);
+  std::vector<uint32_t> v(n);
+  std::iota(v.begin(), v.end(), 0u);
+  std::vector<std::pair<uint32_t, uint32_t>> want(n);
+  for (size_t i = 0; i < n; ++i) want[i] = {k[i], v[i]};
+  std::sort(want.begin(), want.end());
+  MpiComm comm = MpiComm::Virtual(Sizes(p, n, 24));
+  gpusort::SampleSort(comm, &k, &v);
+  bool ok = Partitioned(comm, n) && comm.LastRounds() > 0;
+  for (size_t i = 0; ok && i < n; ++i) ok = k[i] == want[i].first && v[i] == want[i].second;
+  Report("samplesort_small_input_bitonic_p8", ok);
+}
+
 void CaseHyperQuickSort() {
   const size_t p = 8, n = 120000;
   std::vector<int32_t> k = Keys<int32_t>(19, n);
@@ -308,6 +324,7 @@ int main(int argc, char** argv) {
   CaseNonPow2FallsBackToSampleSort();
   CaseSampleSortPairs();
   CaseHyperQuickSort();
+  CaseSampleSortSmallInput();
   CaseGlobalIndexesAndArrange();
   CaseKeepNoProcessHasEmptyData();
   CaseEmptyInput();

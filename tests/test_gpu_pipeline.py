@@ -122,6 +122,43 @@ def test_vrank_matches_cpu_rule(handle, case):
     assert_bits_equal(want_k, keys[order], "oracle keys are sorted")
 
 
+SMALL_CASES = [
+    # algorithm, p, key dtype, payload, descending, distinct keys: N < 5 p^2 takes the bitonic network
+    ("samplesort", 2, np.uint32, True, False, None),
+    ("samplesort", 3, np.uint32, False, False, None),
+    ("samplesort", 5, np.uint64, True, True, None),
+    ("samplesort", 6, np.int32, True, False, 7),
+    ("samplesort", 8, np.float32, False, True, None),
+    ("samplesort", 12, np.uint32, True, False, 3),
+    ("samplesort", 13, np.int64, True, True, None),
+    ("samplesort", 64, np.uint64, True, False, None),
+    ("hyksort", 48, np.uint32, True, False, 50),  # p not a power of 2 -> SampleSort -> small path
+]
+
+
+@pytest.mark.parametrize("case", SMALL_CASES, ids=lambda c: f"{c[0]}-p{c[1]}-{np.dtype(c[2]).name}"
+                         f"{'-kv' if c[3] else ''}{'-desc' if c[4] else ''}{'-d' + str(c[5]) if c[5] else ''}")
+def test_vrank_small_input_bitonic_network(handle, case):
+    algo, p, dtype, kv, desc, distinct = case
+    off = rank_offsets(p, p * 13 + 1, 1, 4 * p)
+    n = int(off[-1])
+    assert n < 5 * p * p
+    keys = keys_of(dtype, n, p + 3 * n, distinct=distinct)
+    vals = np.arange(n, dtype=np.uint32) if kv else None
+    say(f"small vrank {case} n={n}")
+    want_k, want_v, want = oracle.vrank(keys, off.astype(np.int64), vals, algorithm=algo, k_way=8,
+                                        samples_per_rank=32, descending=desc)
+    tk = dev(keys)
+    tv = dev(vals) if kv else None
+    got = handle.vrank_sort(tk, off, tv, algorithm=ALGOS[algo], k_way=8, samples_per_rank=32, descending=desc)
+    assert_bits_equal(host(tk), want_k, "keys")
+    if kv:
+        assert_bits_equal(host(tv), want_v, "payloads")
+    assert got["rounds"] == want["rounds"] > 0
+    assert np.array_equal(got["final_rank_sizes"].astype(np.int64), want["final_rank_sizes"]), "final rank sizes"
+    assert len(got["splitters"]) == 0
+
+
 def test_vrank_empty_rank_rejected(handle):
     from gpusort_amd import GpusortError
 

@@ -284,3 +284,70 @@ def test_multisplit_oracle_matches_searchsorted(upper, descending):
     idx, offs = oracle.multisplit(keys, spl, upper, descending)
     assert np.array_equal(idx, np.argsort(b, kind="stable"))
     assert np.array_equal(offs, np.concatenate([[0], np.cumsum(np.bincount(b, minlength=spl.size + 1))]))
+
+
+def test_merge_lists_known_answers():
+    # Traced by hand through MergeLists (samplesort_helper.h:305-387).
+    a, b = np.arange(1, 9), np.array([9, 10])
+    assert oracle.merge_lists(a, b, True, True).tolist() == list(range(1, 9))      # low = 9 > high = 8
+    assert oracle.merge_lists(b, a, False, True).tolist() == [9, 10]
+    assert oracle.merge_lists([5, 5], [5, 5], True, True).tolist() == [5, 5]
+    assert oracle.merge_lists([5, 5, 5], [5], False, True).tolist() == [5, 5]
+    # decreasing lists: keep low keeps the smallest values (the tail)
+    assert oracle.merge_lists([9, 7, 5], [8, 6, 4], True, False).tolist() == [6, 5, 4]
+    assert oracle.merge_lists([8, 6, 4], [9, 7, 5], False, False).tolist() == [9, 8, 7]
+
+
+def _merge_lists_closed_form(a, b, keep_low, inc):
+    # Each loop condition of MergeLists holds on a prefix (or suffix) of the
+    # sorted merge, so the kept length is a min/max of element counts.
+    s = np.sort(np.concatenate([a, b]))
+    if not inc:
+        s = s[::-1]
+    n, half = s.size, s.size // 2
+    if inc:
+        low, high = max(a[0], b[0]), min(a[-1], b[-1])
+        if keep_low:
+            L = min(max(int((s < low).sum()), half), int((s <= high).sum()))
+            return s[:L]
+        L = max(min(n - half, int((s >= low).sum())), int((s > high).sum()))
+        return s[n - L:]
+    low, high = max(a[-1], b[-1]), min(a[0], b[0])
+    if keep_low:
+        L = max(min(int((s <= high).sum()), n - half), int((s < low).sum()))
+        return s[n - L:]
+    L = max(min(half, int((s >= low).sum())), int((s > high).sum()))
+    return s[:L]
+
+
+def test_merge_lists_equals_closed_form():
+    rng = np.random.default_rng(5)
+    for trial in range(3000):
+        na, nb = rng.integers(1, 12, 2)
+        hi = int(rng.integers(2, 30))
+        a = np.sort(rng.integers(0, hi, na))
+        b = np.sort(rng.integers(0, hi, nb))
+        for inc in (True, False):
+            aa, bb = (a, b) if inc else (a[::-1].copy(), b[::-1].copy())
+            for keep_low in (True, False):
+                got = oracle.merge_lists(aa, bb, keep_low, inc)
+                want = _merge_lists_closed_form(aa, bb, keep_low, inc)
+                assert np.array_equal(got, want), (trial, aa, bb, keep_low, inc)
+
+
+@pytest.mark.parametrize("p", [2, 3, 5, 6, 7, 8, 12, 13])
+def test_small_samplesort_network_invariants(p):
+    # N < 5 p^2: globally sorted, every element kept, no rank empty.
+    rng = np.random.default_rng(p)
+    sizes = rng.integers(1, 4 * p, p)
+    off = np.concatenate([[0], np.cumsum(sizes)]).astype(np.int64)
+    n = int(off[-1])
+    assert n < 5 * p * p
+    k = rng.integers(0, 40, n).astype(np.uint32)
+    v = np.arange(n, dtype=np.uint32)
+    for desc in (False, True):
+        ok, ov, rep = oracle.vrank(k, off, v, algorithm="samplesort", descending=desc)
+        pairs = sorted(zip(k.tolist(), v.tolist()), reverse=desc)
+        assert list(zip(ok.tolist(), ov.tolist())) == pairs
+        assert (rep["final_rank_sizes"] > 0).all() and rep["final_rank_sizes"].sum() == n
+        assert rep["rounds"] > 0 and len(rep["splitters"]) == 0
