@@ -11,6 +11,7 @@
 #include <gpusort/cuda/cuda_utils.h>
 #include <gpusort/hyksort.h>
 #include <gpusort/hyperquicksort.h>
+#include <gpusort/io/rank_files.h>
 #include <gpusort/par/par_utils.h>
 #include <gpusort/samplesort.h>
 #include <stdint.h>
@@ -295,6 +296,37 @@ void CaseEmptyInput() {
   Report("empty_input_throws_reference_exceptions", ok);
 }
 
+// Rank files (no device needed): SaveData's block rule, ReadRanks and
+// SaveRanks round trips; the directory is then read by the Python reader.
+int RankFiles(const std::string& dir) {
+  namespace io = gpusort::io;
+  std::vector<uint32_t> a(10);
+  std::iota(a.begin(), a.end(), 100u);
+  io::SaveData(4, dir, ".dat", a.begin(), a.end());  // blocks of 3: 3, 3, 3, 1
+  std::vector<uint32_t> back;
+  MpiComm c = io::ReadRanks(4, dir, ".dat", &back);
+  Report("rankfiles_save_data_blocks", back == a && c.Sizes() == std::vector<uint64_t>({3, 3, 3, 1}));
+  std::vector<double> d = {0.5, -1.0, 2.25, 8.0, 3.5};
+  MpiComm vc = MpiComm::Virtual({2, 0, 3});
+  io::SaveRanks(vc, dir, ".v1", d);
+  std::vector<double> d2;
+  MpiComm c2 = io::ReadRanks(3, dir, ".v1", &d2);
+  Report("rankfiles_save_ranks_round_trip", d2 == d && c2.Sizes() == vc.Sizes());
+  std::vector<int64_t> b9(9, 7);
+  io::SaveData(4, dir, ".nine", b9.begin(), b9.end());  // the reference's rule writes 2 per rank
+  std::vector<int64_t> r9;
+  MpiComm c9 = io::ReadRanks(4, dir, ".nine", &r9);
+  Report("rankfiles_block_rule_tail", r9.size() == 8 && io::CalcNumElemsEachBlock(9, 4) == 2);
+  bool threw = false;
+  try {
+    io::ReadData(9, dir, ".dat", &back);
+  } catch (const gpusort::MpiException&) {
+    threw = true;
+  }
+  Report("rankfiles_missing_file_throws", threw);
+  return g_failures ? 1 : 0;
+}
+
 int NoDevice() {
   const char* msg = "No CUDA-capable devices were detected";
   Report("no_device_count", gpusort::CudaUtils::GetDeviceCount() == 0);
@@ -310,6 +342,7 @@ int NoDevice() {
 
 int main(int argc, char** argv) {
   if (argc > 1 && std::string(argv[1]) == "nodevice") return NoDevice();
+  if (argc > 2 && std::string(argv[1]) == "rankfiles") return RankFiles(argv[2]);
   CaseCudaUtilsSort<uint32_t>("cudautils_sort_u32_less", std::less<uint32_t>());
   CaseCudaUtilsSort<int32_t>("cudautils_sort_i32_greater", std::greater<int32_t>());
   CaseCudaUtilsSort<float>("cudautils_sort_f32_less", std::less<float>());
