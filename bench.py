@@ -143,6 +143,8 @@ def run_batched(h, g, torch, n_arrays: int, steps: int, seed: int = 4):
     d_off = torch.from_numpy(off.view(np.int64)).cuda()
     torch.cuda.synchronize()
 
+    # Size classes picked by the autotuner on its own synthetic lognormal batch.
+    tuned, table = h.autotune_segments(g.U32, g.U32, total_keys=1 << 26, median=2048.0, sigma=2.0)
     k, v = keys0.clone(), vals0.clone()
     h.sort_segments(k, d_off, v)  # warm-up: grows the workspace
     del k, v
@@ -158,6 +160,27 @@ def run_batched(h, g, torch, n_arrays: int, steps: int, seed: int = 4):
     batched_ms = (t1 - t0) * 1e3 / steps
     bk, bv = copies[-1]
     del copies[:-1]
+
+    # The same batch as a plan replayed from a captured HIP graph: kernel
+    # launches only, no offset read-back, classification or map upload.
+    plan = h.seg_plan(off, g.U32, g.U32)
+    gk, gv = keys0.clone(), vals0.clone()
+    plan.launch_graph(gk, gv)  # capture + first replay
+    torch.cuda.synchronize()
+    graph_same = bool(torch.equal(gk.view(torch.int32), bk.view(torch.int32))) and \
+        bool(torch.equal(gv.view(torch.int32), bv.view(torch.int32)))
+    graph_ms = []
+    for _ in range(steps):
+        gk.copy_(keys0)
+        gv.copy_(vals0)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        plan.launch_graph(gk, gv)
+        torch.cuda.synchronize()
+        graph_ms.append((time.perf_counter() - t0) * 1e3)
+    graph_same = graph_same and bool(torch.equal(gk.view(torch.int32), bk.view(torch.int32)))
+    plan.close()
+    del gk, gv
 
     # One call per array, same buffers (the calls are asynchronous: the host
     # enqueues while the device sorts).
@@ -212,7 +235,11 @@ def run_batched(h, g, torch, n_arrays: int, steps: int, seed: int = 4):
         "one_call_per_array_ms": round(single_ms, 3),
         "one_call_per_array_gkeys_s": round(n / (single_ms * 1e-3) / 1e9, 3),
         "speedup_vs_one_call": round(single_ms / batched_ms, 2),
-        "verified": same and spot,
+        "graph_replay_ms": round(float(np.median(graph_ms)), 3),
+        "graph_replay_gkeys_s": round(n / (float(np.median(graph_ms)) * 1e-3) / 1e9, 3),
+        "tuned_size_classes": tuned,
+        "tuning_table_ms": [(r["small_max"], r["medium_max"], round(r["ms"], 3)) for r in table],
+        "verified": same and spot and graph_same,
         "classes": classes,
         "seg_onesweep_pass": pass_stats,
     }

@@ -116,7 +116,30 @@ class ServerStats(ctypes.Structure):
         "pooled_bytes", "peak_pooled_bytes", "budget")]
 
 
+class SegConfig(ctypes.Structure):
+    _fields_ = [("small_max", ctypes.c_uint32), ("medium_max", ctypes.c_uint64)]
+
+
+class TuneRow(ctypes.Structure):
+    _fields_ = [("config", SegConfig), ("ms", ctypes.c_double)]
+
+
 _SIGS = {
+    "gs_get_seg_config": ([ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.POINTER(SegConfig)], ctypes.c_int),
+    "gs_set_seg_config": ([ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.POINTER(SegConfig)], ctypes.c_int),
+    "gs_autotune_segments": (
+        [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_uint64, ctypes.c_double, ctypes.c_double,
+         ctypes.POINTER(SegConfig), ctypes.POINTER(TuneRow), ctypes.c_uint32, ctypes.POINTER(ctypes.c_uint32)],
+        ctypes.c_int,
+    ),
+    "gs_seg_plan_create": (
+        [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.POINTER(ctypes.c_uint64), ctypes.c_uint32, ctypes.c_int,
+         ctypes.POINTER(ctypes.c_void_p)],
+        ctypes.c_int,
+    ),
+    "gs_seg_plan_run": ([ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p], ctypes.c_int),
+    "gs_seg_plan_launch_graph": ([ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p], ctypes.c_int),
+    "gs_seg_plan_destroy": ([ctypes.c_void_p], ctypes.c_int),
     "gs_multisplit": (
         [ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p, ctypes.c_uint64,
          ctypes.c_void_p, ctypes.c_uint32, ctypes.c_int, ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p,
@@ -342,6 +365,30 @@ class Handle:
             )
         )
 
+    def seg_config(self, kdt: int, vdt: int = NONE) -> dict:
+        c = SegConfig()
+        _check(self.lib.gs_get_seg_config(self.h, kdt, vdt, ctypes.byref(c)))
+        return {"small_max": c.small_max, "medium_max": c.medium_max}
+
+    def set_seg_config(self, kdt: int, vdt: int, small_max: int, medium_max: int) -> None:
+        c = SegConfig(small_max, medium_max)
+        _check(self.lib.gs_set_seg_config(self.h, kdt, vdt, ctypes.byref(c)))
+
+    def autotune_segments(self, kdt: int, vdt: int = NONE, total_keys: int = 1 << 25, median: float = 2048.0,
+                          sigma: float = 2.0):
+        """Times every size-class candidate on a synthetic batch; keeps the best in the handle."""
+        best = SegConfig()
+        rows = (TuneRow * 64)()
+        nr = ctypes.c_uint32()
+        _check(self.lib.gs_autotune_segments(self.h, kdt, vdt, total_keys, median, sigma, ctypes.byref(best), rows,
+                                             64, ctypes.byref(nr)))
+        table = [{"small_max": rows[i].config.small_max, "medium_max": rows[i].config.medium_max,
+                  "ms": rows[i].ms} for i in range(nr.value)]
+        return {"small_max": best.small_max, "medium_max": best.medium_max}, table
+
+    def seg_plan(self, offsets: np.ndarray, kdt: int, vdt: int = NONE, descending: bool = False) -> "SegPlan":
+        return SegPlan(self, offsets, kdt, vdt, descending)
+
     def multisplit(self, keys, splitters: np.ndarray, out_keys, vals=None, out_vals=None, upper: bool = False,
                    descending: bool = False) -> np.ndarray:
         """Stable multisplit into len(splitters) + 1 buckets; returns the bucket offsets."""
@@ -450,6 +497,37 @@ class Handle:
                 int(descending), None if pairs_out is None else pairs_out.ctypes.data, pair_stride, val_offset,
             )
         )
+
+
+class SegPlan:
+    """Batched sort planned for fixed segment offsets (gs_seg_plan_*): run()
+    queues kernel launches only; launch_graph() replays a captured HIP graph."""
+
+    def __init__(self, handle: "Handle", offsets: np.ndarray, kdt: int, vdt: int, descending: bool):
+        self.handle = handle
+        self.lib = handle.lib
+        off = np.ascontiguousarray(offsets, dtype=np.uint64)
+        p = ctypes.c_void_p()
+        _check(self.lib.gs_seg_plan_create(handle.h, kdt, vdt, off.ctypes.data_as(ctypes.POINTER(ctypes.c_uint64)),
+                                           off.size - 1, int(descending), ctypes.byref(p)))
+        self.p = p.value
+
+    def run(self, keys, vals=None) -> None:
+        _check(self.lib.gs_seg_plan_run(self.p, _dev_ptr(keys), None if vals is None else _dev_ptr(vals)))
+
+    def launch_graph(self, keys, vals=None) -> None:
+        _check(self.lib.gs_seg_plan_launch_graph(self.p, _dev_ptr(keys), None if vals is None else _dev_ptr(vals)))
+
+    def close(self) -> None:
+        if getattr(self, "p", None):
+            self.lib.gs_seg_plan_destroy(self.p)
+            self.p = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 class Server:

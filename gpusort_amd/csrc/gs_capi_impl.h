@@ -6,6 +6,7 @@
 #include <stddef.h>
 #include <stdint.h>
 
+#include <map>
 #include <string>
 #include <vector>
 
@@ -87,7 +88,7 @@ gs_status sort_device(gs_handle* h, gs_dtype kdt, void* d_keys, gs_dtype vdt, vo
 // in `huge` for the caller to sort.
 gs_status sort_segments_async(gs_handle* h, gs_dtype kdt, void* d_keys, gs_dtype vdt, void* d_vals,
                               const uint64_t* off, uint32_t nseg, int descending, std::vector<unsigned char>* meta,
-                              std::vector<uint32_t>* huge);
+                              std::vector<uint32_t>* huge, const SegConfig* config = nullptr);
 
 }  // namespace gs
 
@@ -99,4 +100,11 @@ struct gs_handle {
   gs::LaunchTimer* timer() { return profiler.active ? &profiler : nullptr; }
   void* pinned = nullptr;
   size_t pinned_bytes = 0;
+  // Size-class configuration of the batched sort per (key, payload) width,
+  // set by the autotuner; defaults otherwise.
+  std::map<int, gs::SegConfig> seg_tuned;
+  gs::SegConfig seg_config(int kb, int vb) const {
+    auto it = seg_tuned.find(kb * 16 + vb);
+    return it == seg_tuned.end() ? gs::default_seg_config(kb, vb) : it->second;
+  }
 };

@@ -70,6 +70,7 @@ uint32_t radix_tile_keys(int key_bytes, int val_bytes);
 // ping-pong buffer in the workspace mirrors the whole array); n_keys is the
 // number of keys in the segments.
 struct SegTiles {
+  int geo;  // tile geometry: 0 = radix_tile_keys(), 1 = the small one (radix_seg_tile_keys)
   const uint32_t* tile_seg;
   const uint32_t* seg_first;
   const uint64_t* seg_range;
@@ -89,6 +90,19 @@ hipError_t multisplit(int key_bytes, int val_bytes, const void* keys, const void
                       const void* d_splitters, uint32_t nspl, int upper, const XformParams& xf, void* out_keys,
                       void* out_vals, uint64_t* d_bucket_base, void* workspace, size_t workspace_bytes,
                       hipStream_t stream, LaunchTimer* timer = nullptr);
+
+// Keys per tile of segmented-sort geometry `geo` (0 or 1).
+uint32_t radix_seg_tile_keys(int key_bytes, int val_bytes, int geo);
+
+// Size classes of the batched segmented sort: segments of at most small_max
+// keys (<= block_sort_max) are sorted by the LDS bitonic kernel, those of at
+// most medium_max by the segmented onesweep with the small tiles, the rest
+// with the large tiles.
+struct SegConfig {
+  uint32_t small_max;
+  uint64_t medium_max;
+};
+SegConfig default_seg_config(int key_bytes, int val_bytes);
 
 size_t radix_segments_workspace_bytes(int key_bytes, int val_bytes, uint64_t n_total, uint64_t tiles,
                                       uint32_t nslots);

@@ -539,6 +539,24 @@ static inline uint32_t tile_keys_for(int kb, int vb) {
 
 uint32_t radix_tile_keys(int key_bytes, int val_bytes) { return tile_keys_for(key_bytes, val_bytes); }
 
+// Tile geometries of the segmented sort: 0 = the single-array geometry above,
+// 1 = 256 threads (4K keys for 32-bit keys with 0/4-byte payloads, else 3K),
+// for segments of a few tiles: less of the last tile is padding and up to
+// four workgroups share a CU.
+template <typename U, int VB, int G>
+struct SegGeometry {
+  static constexpr int THREADS = Geometry<U, VB>::THREADS, ITEMS = Geometry<U, VB>::ITEMS;
+};
+template <typename U, int VB>
+struct SegGeometry<U, VB, 1> {
+  static constexpr int THREADS = 256, ITEMS = (sizeof(U) == 4 && VB <= 4) ? 16 : 12;
+};
+
+uint32_t radix_seg_tile_keys(int kb, int vb, int geo) {
+  if (geo == 0) return tile_keys_for(kb, vb);
+  return (kb == 4 && vb <= 4) ? 256 * 16 : 256 * 12;
+}
+
 struct RadixLayout {
   size_t off_keys_tmp, off_vals_tmp, off_ctrl, off_hist, off_base, off_status, total;
   uint64_t tiles;
@@ -767,13 +785,13 @@ size_t radix_segments_workspace_bytes(int key_bytes, int val_bytes, uint64_t n_t
   return seg_layout(key_bytes, val_bytes, n_total, tiles, nslots).total;
 }
 
-template <typename U, int VB, typename Idx>
+template <typename U, int VB, typename Idx, int G>
 static hipError_t radix_segments_t(U* keys, void* vals_v, const SegTiles& T, const XformParams& xp,
                                    unsigned char* ws, const SegLayout& L, hipStream_t stream, LaunchTimer* timer) {
   using V = typename ValT<VB>::T;
   constexpr bool KV = VB != 0;
-  constexpr int THREADS = Geometry<U, VB>::THREADS;
-  constexpr int ITEMS = Geometry<U, VB>::ITEMS;
+  constexpr int THREADS = SegGeometry<U, VB, G>::THREADS;
+  constexpr int ITEMS = SegGeometry<U, VB, G>::ITEMS;
   const Xform<U> xf = xform_of<U>(xp);
   const bool ident = xf.float_mask == 0 && xf.sign_flip == 0 && xf.desc_mask == 0;
   SortCtrl* ctrl = reinterpret_cast<SortCtrl*>(ws + L.off_ctrl);
@@ -835,8 +853,13 @@ static hipError_t radix_segments_t(U* keys, void* vals_v, const SegTiles& T, con
 template <typename U, int VB>
 static hipError_t radix_segments_i(U* keys, void* vals, const SegTiles& T, const XformParams& xp, unsigned char* ws,
                                    const SegLayout& L, hipStream_t stream, LaunchTimer* timer) {
-  if (T.n_total < (1ull << 32)) return radix_segments_t<U, VB, uint32_t>(keys, vals, T, xp, ws, L, stream, timer);
-  return radix_segments_t<U, VB, uint64_t>(keys, vals, T, xp, ws, L, stream, timer);
+  if (T.geo == 1) {
+    if (T.n_total < (1ull << 32))
+      return radix_segments_t<U, VB, uint32_t, 1>(keys, vals, T, xp, ws, L, stream, timer);
+    return radix_segments_t<U, VB, uint64_t, 1>(keys, vals, T, xp, ws, L, stream, timer);
+  }
+  if (T.n_total < (1ull << 32)) return radix_segments_t<U, VB, uint32_t, 0>(keys, vals, T, xp, ws, L, stream, timer);
+  return radix_segments_t<U, VB, uint64_t, 0>(keys, vals, T, xp, ws, L, stream, timer);
 }
 
 hipError_t radix_sort_segments(int kb, int vb, void* keys, void* vals, const SegTiles& T, const XformParams& xf,

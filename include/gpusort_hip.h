@@ -190,6 +190,52 @@ typedef struct gs_kernel_stat {
 gs_status gs_profile_begin(gs_handle* h);
 gs_status gs_profile_end(gs_handle* h, gs_kernel_stat* out, uint32_t max_out, uint32_t* n_out);
 
+/* Size classes of the batched sort for one (key, payload) width pair:
+ * segments of at most small_max keys (at most the LDS tile) are sorted by the
+ * LDS bitonic kernel, those of at most medium_max by the segmented onesweep
+ * with small (4K / 3K-key) tiles, the rest with the large tiles.  Kept per
+ * handle; gs_autotune_segments picks them by measurement.  No counterpart in
+ * the reference (one array per call, src/cuda_utils.cu:97-243). */
+typedef struct gs_seg_config {
+  uint32_t small_max;
+  uint64_t medium_max;
+} gs_seg_config;
+gs_status gs_get_seg_config(gs_handle* h, gs_dtype kdt, gs_dtype vdt, gs_seg_config* out);
+gs_status gs_set_seg_config(gs_handle* h, gs_dtype kdt, gs_dtype vdt, const gs_seg_config* cfg);
+
+/* One measured candidate of the autotuner. */
+typedef struct gs_tune_row {
+  gs_seg_config config;
+  double ms; /* median time of the batched sort */
+} gs_tune_row;
+
+/* Autotuner of the batched sort: builds a synthetic batch of about
+ * total_keys keys in arrays with lognormal lengths (median, sigma; clipped to
+ * [1, 2^22]), uniform keys and index payloads of the given types, times every
+ * candidate size-class configuration (LDS-tile limit x small-tile limit) with
+ * events, stores the fastest in the handle and returns it (and every
+ * candidate's time in rows, up to max_rows).  Synchronous. */
+gs_status gs_autotune_segments(gs_handle* h, gs_dtype kdt, gs_dtype vdt, uint64_t total_keys, double median,
+                               double sigma, gs_seg_config* best, gs_tune_row* rows, uint32_t max_rows,
+                               uint32_t* n_rows);
+
+/* A plan of the batched sort for fixed host segment offsets, types and
+ * direction (with the handle's size classes at creation): the device maps and
+ * workspace are built once and owned by the plan, so running it is kernel
+ * launches only -- no host copies, allocation or synchronisation -- and can
+ * be captured into a HIP graph.  Segments must be shorter than 2^30 keys.
+ * Runs on the handle's stream (or the plan's own stream when the handle has
+ * none); calls on one plan must not overlap. */
+typedef struct gs_seg_plan gs_seg_plan;
+gs_status gs_seg_plan_create(gs_handle* h, gs_dtype kdt, gs_dtype vdt, const uint64_t* h_offsets, uint32_t nseg,
+                             int descending, gs_seg_plan** out);
+/* Queues the planned sort of d_keys / d_vals (asynchronous). */
+gs_status gs_seg_plan_run(gs_seg_plan* p, void* d_keys, void* d_vals);
+/* Captures the planned sort of these buffers into a HIP graph on first use
+ * (again when the buffers change) and launches the graph (asynchronous). */
+gs_status gs_seg_plan_launch_graph(gs_seg_plan* p, void* d_keys, void* d_vals);
+gs_status gs_seg_plan_destroy(gs_seg_plan* p);
+
 /* Stable multisplit of unsorted device keys (and payloads) into nspl + 1
  * buckets, nspl <= 255, by the sorted splitters h_splitters (host, key type,
  * sorted in the direction's order): a key's bucket is the number of

@@ -110,3 +110,72 @@ def test_repeat_calls_shrinking(handle):
     # Workspace and maps are reused: a second, smaller call must not see the first.
     run_case(handle, np.uint32, np.uint32, [70000, 20000, 5000, 100], 27, descending=False)
     run_case(handle, np.uint32, np.uint32, [17000, 3], 28, descending=False)
+
+
+U32 = 1
+
+
+@pytest.mark.parametrize("small_max,medium_max", [(4096, 0), (1024, 1 << 16), (256, 1 << 20)],
+                         ids=["default", "small-tiles-64K", "small-tiles-1M"])
+def test_size_class_configs_agree(handle, small_max, medium_max):
+    # Every size-class configuration sorts identically.
+    handle.set_seg_config(U32, U32, small_max, medium_max)
+    try:
+        assert handle.seg_config(U32, U32) == {"small_max": small_max, "medium_max": medium_max}
+        run_case(handle, np.uint32, np.uint32, BOUNDARY_LENGTHS, 31, descending=False)
+        lengths = datagen.lognormal_lengths(7, 2000, median=2048, sigma=2.0, lo=1, hi=1 << 20)
+        run_case(handle, np.uint32, np.uint32, lengths, 32, descending=True)
+    finally:
+        handle.set_seg_config(U32, U32, 4096, 0)
+
+
+def test_autotune_picks_the_fastest_candidate(handle):
+    say("autotune u32+u32 on 2^22 keys")
+    best, table = handle.autotune_segments(U32, U32, total_keys=1 << 22)
+    try:
+        assert len(table) == 9 and all(r["ms"] > 0 for r in table)
+        fastest = min(table, key=lambda r: r["ms"])
+        assert best == {"small_max": fastest["small_max"], "medium_max": fastest["medium_max"]}
+        assert handle.seg_config(U32, U32) == best
+        lengths = datagen.lognormal_lengths(8, 1500, median=2048, sigma=2.0, lo=1, hi=1 << 20)
+        run_case(handle, np.uint32, np.uint32, lengths, 33, descending=False)
+    finally:
+        handle.set_seg_config(U32, U32, 4096, 0)
+
+
+def test_plan_run_and_graph_replays(handle):
+    import torch
+
+    lengths = datagen.lognormal_lengths(9, 1500, median=2048, sigma=2.0, lo=1, hi=1 << 20)
+    off = offsets_of(lengths)
+    n = int(off[-1])
+    keys = keys_of(np.uint32, n, 34)
+    vals = np.arange(n, dtype=np.uint32)
+    wk, wv = oracle.seg_stable_sort_pairs(keys, vals, off.astype(np.int64), False)
+    plan = handle.seg_plan(off, U32, U32)
+    say(f"plan run n={n}")
+    tk, tv = dev(keys), dev(vals)
+    plan.run(tk, tv)
+    torch.cuda.synchronize()
+    assert_bits_equal(host(tk), wk, "plan keys")
+    assert_bits_equal(host(tv), wv, "plan payloads")
+    src_k, src_v = dev(keys), dev(vals)
+    gk, gv = torch.empty_like(src_k), torch.empty_like(src_v)
+    other = dev(keys_of(np.uint32, 1 << 16, 35))
+    for replay in range(3):
+        gk.copy_(src_k)
+        gv.copy_(src_v)
+        torch.cuda.synchronize()
+        say(f"graph replay {replay}")
+        plan.launch_graph(gk, gv)
+        torch.cuda.synchronize()
+        assert_bits_equal(host(gk), wk, f"graph replay {replay} keys")
+        assert_bits_equal(host(gv), wv, f"graph replay {replay} payloads")
+        handle.sort_keys(other)  # eager work between replays
+        torch.cuda.synchronize()
+    # new buffers: the plan captures a new graph
+    hk, hv = dev(keys), dev(vals)
+    plan.launch_graph(hk, hv)
+    torch.cuda.synchronize()
+    assert_bits_equal(host(hk), wk, "recaptured keys")
+    plan.close()
