@@ -489,7 +489,9 @@ template <typename U, int VB>
 struct Geometry;
 // Tile sizes: 32-bit keys with no or 32-bit payloads use 1024 threads x 18
 // keys (18K-key tiles: fewer look-back steps per key; 18 keys is the most
-// that fits 128 VGPRs without spills), the rest 512 x 12.
+// that fits 128 VGPRs without spills), 64-bit keys with no or 32-bit
+// payloads 1024 x 10 (measured best of 512x12, 512x16, 768x12, 1024x8/10/12),
+// 8-byte payloads 512 x 12.
 // LDS holds one tile of the wider of key and payload, reused for the payload
 // scatter.
 template <>
@@ -506,11 +508,11 @@ struct Geometry<uint32_t, 8> {
 };
 template <>
 struct Geometry<uint64_t, 0> {
-  static constexpr int THREADS = 512, ITEMS = 12;
+  static constexpr int THREADS = 1024, ITEMS = 10;
 };
 template <>
 struct Geometry<uint64_t, 4> {
-  static constexpr int THREADS = 512, ITEMS = 12;
+  static constexpr int THREADS = 1024, ITEMS = 10;
 };
 template <>
 struct Geometry<uint64_t, 8> {
