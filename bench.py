@@ -473,6 +473,98 @@ def run_c3(h, g, torch, with_cpu: bool, steps: int = 3):
     return out
 
 
+def run_served_host(h, g, torch, n_requests: int, seed: int = 6, loads=(0.5, 1.5)):
+    """Served requests whose data is in (pinned) host memory: each request is
+    copied in, sorted and copied back by the server (PCIe included).  Sizes
+    log-uniform over [2^10, 2^26] keys, same dtype mix as the device-resident
+    leg; serial server (one request at a time: copy in, sort, copy out) vs
+    served (4 slots, copies of one slot overlapping the others' sorts, small
+    requests coalesced)."""
+    from gpusort_amd import datagen
+
+    sizes = datagen.loguniform_sizes(seed, n_requests, lo=1 << 10, hi=1 << 26)
+    u = (datagen.draws(seed + 1, n_requests) >> np.uint64(11)).astype(np.float64) / float(1 << 53)
+    kinds = np.where(u < 0.5, 0, np.where(u < 0.8, 1, 2))
+    total = int(sizes.sum())
+    kdt = {0: torch.uint32, 1: torch.uint64, 2: torch.float32}
+    dev_in, host, expect = [], [], []
+    for i in range(n_requests):
+        n = int(sizes[i])
+        k = torch.empty(n, dtype=kdt[int(kinds[i])], device="cuda")
+        v = None
+        if kinds[i] == 0:
+            h.generate(k, g.GEN_UNIFORM_U32, 2000 + i)
+        elif kinds[i] == 1:
+            h.generate(k, g.GEN_UNIFORM_U64, 2000 + i)
+            v = torch.empty(n, dtype=torch.uint32, device="cuda")
+            h.generate(v, g.GEN_IOTA_U32, 0)
+        else:
+            h.generate(k, g.GEN_NORMAL_F32, 2000 + i, param=1e3)
+        ek, ev = k.clone(), None if v is None else v.clone()
+        if ev is None:
+            h.sort_keys(ek)
+        else:
+            h.sort_pairs(ek, ev)
+        dev_in.append((k, v))
+        expect.append((ek, ev))
+        host.append((torch.empty(n, dtype=k.dtype, pin_memory=True),
+                     None if v is None else torch.empty(n, dtype=v.dtype, pin_memory=True)))
+    torch.cuda.synchronize()
+
+    def as_int(t):
+        return t.view(torch.int32) if t.element_size() == 4 else t.view(torch.int64)
+
+    def one_run(serial: bool, rate):
+        for (k, v), (hk, hv) in zip(dev_in, host):
+            hk.copy_(k)
+            if v is not None:
+                hv.copy_(v)
+        torch.cuda.synchronize()
+        if rate is None:
+            arrivals = np.zeros(n_requests)
+        else:
+            gaps = -np.log(1.0 - (datagen.draws(seed + 2, n_requests) >> np.uint64(11)).astype(np.float64)
+                           / float(1 << 53)) / rate
+            arrivals = np.cumsum(gaps) - gaps[0]
+        srv = g.Server(serial=serial, streams=4, hbm_budget=200 * 10**9, coalesce_max_keys=1 << 16)
+        t0 = time.perf_counter()
+        ids = []
+        for i, (hk, hv) in enumerate(host):
+            while time.perf_counter() - t0 < arrivals[i]:
+                pass
+            ids.append(srv.submit_host(hk, hv))
+        srv.drain()
+        infos = [srv.info(r) for r in ids]
+        srv.close()
+        lat = np.array([(x["done_us"] - x["submit_us"]) * 1e-3 for x in infos])
+        span_s = (max(x["done_us"] for x in infos) - min(x["submit_us"] for x in infos)) * 1e-6
+        exact = 0
+        for (hk, hv), (ek, ev) in zip(host, expect):
+            ok = bool(torch.equal(as_int(hk), as_int(ek.cpu())))
+            if hv is not None:
+                ok = ok and bool(torch.equal(as_int(hv), as_int(ev.cpu())))
+            exact += int(ok)
+        return {"gkeys_s": round(total / span_s / 1e9, 3), "p50_ms": round(float(np.percentile(lat, 50)), 3),
+                "p99_ms": round(float(np.percentile(lat, 99)), 3), "makespan_ms": round(span_s * 1e3, 3),
+                "bit_exact": f"{exact}/{n_requests}"}
+
+    out = {"requests": n_requests, "keys": total, "sizes": "log-uniform 2^10..2^26",
+           "mix": {"u32": int((kinds == 0).sum()), "u64+u32": int((kinds == 1).sum()), "f32": int((kinds == 2).sum())}}
+    serial_sat = one_run(True, None)
+    served_sat = one_run(False, None)
+    out["saturation"] = {"one_call": serial_sat, "served": served_sat,
+                         "speedup": round(served_sat["gkeys_s"] / serial_sat["gkeys_s"], 2)}
+    cap = n_requests / (serial_sat["makespan_ms"] * 1e-3)
+    out["loads"] = []
+    for f in loads:
+        rate = f * cap
+        out["loads"].append({"offered_vs_one_call_capacity": f,
+                             "one_call": one_run(True, rate), "served": one_run(False, rate)})
+    del dev_in, host, expect
+    torch.cuda.empty_cache()
+    return out
+
+
 def roofline_of(prof: dict, name: str = "onesweep_pass"):
     st = prof.get(name)
     if not st or st["launches"] == 0:
@@ -633,6 +725,8 @@ def main() -> int:
         extra["batched_arrays_kv_u32"] = run_batched(h, g, torch, args.arrays, 3)
         log(f"[bench] served: {args.requests} requests, serial vs served, Poisson load sweep")
         extra["served_mixed"] = run_served(h, g, torch, args.requests)
+        log("[bench] served from host memory: 100 requests, serial vs served")
+        extra["served_host_memory"] = run_served_host(h, g, torch, 100)
         log("[bench] KV f32 normal 2^27")
         r = run_workload(h, g, torch, "normal_f32", n // 2, 3, 1, payload=True, profile=False)
         extra["kv_f32_normal_2^27"] = {"gkeys_s": round(r["gkeys_s"], 3), "ms": round(r["ms_per_sort"], 3),

@@ -152,6 +152,11 @@ _SIGS = {
          ctypes.POINTER(ctypes.c_uint64)],
         ctypes.c_int,
     ),
+    "gs_server_submit_host": (
+        [ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p, ctypes.c_uint64, ctypes.c_int,
+         ctypes.POINTER(ctypes.c_uint64)],
+        ctypes.c_int,
+    ),
     "gs_server_wait": ([ctypes.c_void_p, ctypes.c_uint64, ctypes.c_int64], ctypes.c_int),
     "gs_server_drain": ([ctypes.c_void_p], ctypes.c_int),
     "gs_server_info": ([ctypes.c_void_p, ctypes.c_uint64, ctypes.POINTER(RequestInfo)], ctypes.c_int),
@@ -551,6 +556,18 @@ class Server:
             self.s, torch_dtype_code(keys), _dev_ptr(keys), NONE if vals is None else torch_dtype_code(vals),
             None if vals is None else _dev_ptr(vals), keys.numel(), int(descending), ctypes.byref(rid)))
         self._live[rid.value] = (keys, vals)  # keep the buffers alive until done
+        return rid.value
+
+    def submit_host(self, keys, vals=None, descending: bool = False) -> int:
+        """Queues a request in host memory (torch CPU tensors, pinned for
+        overlapping copies); sorted in place."""
+        if keys.is_cuda or (vals is not None and vals.is_cuda):
+            raise GpusortError(1, "submit_host takes host tensors")
+        rid = ctypes.c_uint64()
+        _check(self.lib.gs_server_submit_host(
+            self.s, torch_dtype_code(keys), keys.data_ptr(), NONE if vals is None else torch_dtype_code(vals),
+            None if vals is None else vals.data_ptr(), keys.numel(), int(descending), ctypes.byref(rid)))
+        self._live[rid.value] = (keys, vals)
         return rid.value
 
     def wait(self, rid: int, timeout_us: int = -1) -> bool:

@@ -14,9 +14,11 @@
 //   - runs every other request as its own sort on its slot's stream, so
 //     requests that arrive while others run overlap on the device;
 //   - marks requests done when their slot's event completes.
-// Requests are device-resident: the caller owns the key/payload buffers and
-// must not touch them until the request is done.  Results are exactly those
-// of gs_sort_keys / gs_sort_pairs on the same input.
+// Requests are device-resident (sorted in place) or in host memory (copied
+// into the slot's staging buffers, sorted, copied back; pinned host memory
+// lets one slot's copies overlap the others' sorts).  The caller must not
+// touch a request's buffers until it is done.  Results are exactly those of
+// gs_sort_keys / gs_sort_pairs on the same input.
 //
 // serial = 1 turns the same server into the one-call-per-request baseline:
 // one slot, no coalescing, one request at a time.
@@ -68,8 +70,9 @@ hipError_t copy_jobs(int elem_bytes, const CopyJob* d_jobs, uint32_t njobs, hipS
 struct Request {
   uint64_t id = 0;
   gs_dtype kdt = GS_U32, vdt = GS_NONE;
-  void* keys = nullptr;
+  void* keys = nullptr;  // device memory, or host memory when `host`
   void* vals = nullptr;
+  bool host = false;
   uint64_t n = 0;
   int desc = 0;
   double submit_us = 0, start_us = -1, done_us = -1;
@@ -122,10 +125,11 @@ namespace {
 // Pool bytes a batch may add to a slot, an upper bound that follows the
 // arena's growth rule slot by slot: one request sorted on its own, or
 // `count` requests of `total` keys coalesced.
-size_t need_single(gs_dtype kdt, gs_dtype vdt, uint64_t n) {
+size_t need_single(gs_dtype kdt, gs_dtype vdt, uint64_t n, bool host) {
   const int kb = dtype_bytes(kdt), vb = val_bytes_of(vdt);
-  if (n <= block_sort_max(kb, vb)) return 0;
-  return Arena::grown(radix_workspace_bytes(kb, vb, n));
+  size_t b = n <= block_sort_max(kb, vb) ? 0 : Arena::grown(radix_workspace_bytes(kb, vb, n));
+  if (host) b += Arena::grown((size_t)n * kb) + (vb ? Arena::grown((size_t)n * vb) : 0);  // device staging
+  return b;
 }
 
 size_t need_coalesced(gs_dtype kdt, gs_dtype vdt, uint64_t total, uint64_t count) {
@@ -146,9 +150,45 @@ gs_status gs_server::launch(Slot& slot, const std::vector<Request>& batch) {
   gs_handle* h = slot.h;
   const Request& r0 = batch[0];
   DeviceScope ds(h->device);
-  if (batch.size() == 1) {
+  if (batch.size() == 1 && !r0.host) {
     gs_status s = sort_device(h, r0.kdt, r0.keys, r0.vdt, r0.vals, r0.n, r0.desc);
     if (s != GS_OK) return s;
+    return from_hip(hipEventRecord(slot.ev, h->stream), "record completion");
+  }
+  if (r0.host) {
+    // Host requests: copied into the slot's staging buffers (back to back for
+    // a coalesced batch), sorted there, copied back.  With pinned host
+    // memory the copies of one slot overlap the sorts of the others.
+    const int kb = dtype_bytes(r0.kdt), vb = val_bytes_of(r0.vdt);
+    const uint32_t nreq = (uint32_t)batch.size();
+    std::vector<uint64_t> off(nreq + 1, 0);
+    for (uint32_t i = 0; i < nreq; ++i) off[i + 1] = off[i] + batch[i].n;
+    void *sk = nullptr, *sv = nullptr;
+    hipError_t e = h->arena.get(Arena::kStageKeys, off[nreq] * kb, h->stream, &sk);
+    if (e == hipSuccess && vb) e = h->arena.get(Arena::kStageVals, off[nreq] * vb, h->stream, &sv);
+    if (e != hipSuccess) return from_hip(e, "server staging");
+    char* ck = reinterpret_cast<char*>(sk);
+    char* cv = reinterpret_cast<char*>(sv);
+    for (uint32_t i = 0; i < nreq && e == hipSuccess; ++i) {
+      e = hipMemcpyAsync(ck + off[i] * kb, batch[i].keys, batch[i].n * kb, hipMemcpyHostToDevice, h->stream);
+      if (e == hipSuccess && vb)
+        e = hipMemcpyAsync(cv + off[i] * vb, batch[i].vals, batch[i].n * vb, hipMemcpyHostToDevice, h->stream);
+    }
+    if (e != hipSuccess) return from_hip(e, "server copy in");
+    gs_status s = GS_OK;
+    if (nreq == 1) {
+      s = sort_device(h, r0.kdt, sk, r0.vdt, sv, r0.n, r0.desc);
+    } else {
+      std::vector<uint32_t> huge;
+      s = sort_segments_async(h, r0.kdt, sk, r0.vdt, sv, off.data(), nreq, r0.desc, &slot.meta, &huge);
+    }
+    if (s != GS_OK) return s;
+    for (uint32_t i = 0; i < nreq && e == hipSuccess; ++i) {
+      e = hipMemcpyAsync(batch[i].keys, ck + off[i] * kb, batch[i].n * kb, hipMemcpyDeviceToHost, h->stream);
+      if (e == hipSuccess && vb)
+        e = hipMemcpyAsync(batch[i].vals, cv + off[i] * vb, batch[i].n * vb, hipMemcpyDeviceToHost, h->stream);
+    }
+    if (e != hipSuccess) return from_hip(e, "server copy out");
     return from_hip(hipEventRecord(slot.ev, h->stream), "record completion");
   }
   // Coalesced: gather -> segmented sort -> scatter.
@@ -205,7 +245,9 @@ bool gs_server::admit_one(Slot& slot, std::unique_lock<std::mutex>& lk) {
     uint64_t keys = 0;
     for (uint64_t id : pending) {
       const Request& r = reqs[id];
-      if (r.n > cfg.coalesce_max_keys || r.kdt != head.kdt || r.vdt != head.vdt || r.desc != head.desc) continue;
+      if (r.n > cfg.coalesce_max_keys || r.kdt != head.kdt || r.vdt != head.vdt || r.desc != head.desc ||
+          r.host != head.host)
+        continue;
       const size_t trial = need_coalesced(head.kdt, head.vdt, keys + r.n, ids.size() + 1);
       if (!ids.empty() && (keys + r.n > cfg.batch_max_keys || ids.size() >= cfg.batch_max_requests ||
                            trial > cfg.hbm_budget))
@@ -216,7 +258,7 @@ bool gs_server::admit_one(Slot& slot, std::unique_lock<std::mutex>& lk) {
     }
   } else {
     ids.push_back(pending.front());
-    need = need_single(head.kdt, head.vdt, head.n);
+    need = need_single(head.kdt, head.vdt, head.n, head.host);
   }
   // Memory: the slot may grow by `need`; idle pools are released first when
   // the budget would be exceeded; otherwise the batch waits for busy slots.
@@ -346,8 +388,8 @@ gs_status gs_server_create(gs_server** out, const gs_server_config* cfg) {
   return GS_OK;
 }
 
-gs_status gs_server_submit(gs_server* s, gs_dtype kdt, void* d_keys, gs_dtype vdt, void* d_vals, uint64_t n,
-                           int descending, uint64_t* id) {
+static gs_status server_submit(gs_server* s, gs_dtype kdt, void* d_keys, gs_dtype vdt, void* d_vals, uint64_t n,
+                               int descending, bool host, uint64_t* id) {
   if (!s || !id) return fail(GS_INVALID_ARG, "null argument");
   if (!dtype_ok(kdt)) return fail(GS_INVALID_ARG, "unsupported key type %d", (int)kdt);
   if (vdt != GS_NONE && !dtype_ok(vdt)) return fail(GS_INVALID_ARG, "unsupported value type %d", (int)vdt);
@@ -359,9 +401,10 @@ gs_status gs_server_submit(gs_server* s, gs_dtype kdt, void* d_keys, gs_dtype vd
   r.vals = vdt == GS_NONE ? nullptr : d_vals;
   r.n = n;
   r.desc = descending != 0;
+  r.host = host;
   {
     const bool small = !s->cfg.serial && n <= s->cfg.coalesce_max_keys;
-    const size_t need = small ? need_coalesced(kdt, vdt, n, 1) : need_single(kdt, vdt, n);
+    const size_t need = small ? need_coalesced(kdt, vdt, n, 1) : need_single(kdt, vdt, n, host);
     if (need > s->cfg.hbm_budget) {
       std::lock_guard<std::mutex> lk(s->mu);
       s->st.refused++;
@@ -384,6 +427,16 @@ gs_status gs_server_submit(gs_server* s, gs_dtype kdt, void* d_keys, gs_dtype vd
   *id = r.id;
   s->cv_work.notify_one();
   return GS_OK;
+}
+
+gs_status gs_server_submit(gs_server* s, gs_dtype kdt, void* d_keys, gs_dtype vdt, void* d_vals, uint64_t n,
+                           int descending, uint64_t* id) {
+  return server_submit(s, kdt, d_keys, vdt, d_vals, n, descending, false, id);
+}
+
+gs_status gs_server_submit_host(gs_server* s, gs_dtype kdt, void* h_keys, gs_dtype vdt, void* h_vals, uint64_t n,
+                                int descending, uint64_t* id) {
+  return server_submit(s, kdt, h_keys, vdt, h_vals, n, descending, true, id);
 }
 
 gs_status gs_server_wait(gs_server* s, uint64_t id, int64_t timeout_us) {

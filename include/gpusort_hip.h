@@ -314,6 +314,12 @@ gs_status gs_server_create(gs_server** out, const gs_server_config* cfg);
 /* Queues a request; GS_ADMISSION_REFUSED when its workspace alone exceeds the budget. */
 gs_status gs_server_submit(gs_server* s, gs_dtype kdt, void* d_keys, gs_dtype vdt, void* d_vals, uint64_t n,
                            int descending, uint64_t* id);
+/* Queues a request whose keys / payloads are in host memory: they are copied
+ * into the server's pooled device staging, sorted and copied back in place.
+ * Pinned host memory (hipHostMalloc) lets copies overlap other requests'
+ * sorts; pageable memory works with synchronous copies. */
+gs_status gs_server_submit_host(gs_server* s, gs_dtype kdt, void* h_keys, gs_dtype vdt, void* h_vals, uint64_t n,
+                                int descending, uint64_t* id);
 /* Waits for one request (timeout_us < 0: no limit); GS_NOT_READY on timeout. */
 gs_status gs_server_wait(gs_server* s, uint64_t id, int64_t timeout_us);
 /* Waits for every submitted request. */

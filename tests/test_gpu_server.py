@@ -115,3 +115,63 @@ def test_wait_and_timeout(handle):
         handle.sort_pairs(k0, v0)
         torch.cuda.synchronize()
         assert same(k, k0) and same(v, v0)
+
+
+def pinned_copy(t):
+    h = torch.empty(t.shape, dtype=t.dtype, pin_memory=True)
+    h.copy_(t)
+    return h
+
+
+@pytest.mark.parametrize("serial", [False, True], ids=["served", "serial"])
+def test_host_requests_equal_alone(handle, serial):
+    sizes = sizes_mix(4, 40)
+    reqs = []
+    for i, n in enumerate(sizes):
+        kind = KINDS[i % len(KINDS)]
+        k, v = make_request(handle, kind, int(n), 4000 + i)
+        torch.cuda.synchronize()
+        hk = pinned_copy(k)
+        hv = None if v is None else pinned_copy(v)
+        desc = i % 3 == 0
+        if v is None:
+            handle.sort_keys(k, descending=desc)
+        else:
+            handle.sort_pairs(k, v, descending=desc)
+        reqs.append((hk, hv, k, v, desc))
+    torch.cuda.synchronize()
+    say(f"host requests serial={serial} count={len(reqs)}")
+    with g.Server(streams=4, serial=serial) as srv:
+        ids = [srv.submit_host(hk, hv, desc) for (hk, hv, _, _, desc) in reqs]
+        srv.drain()
+        st = srv.stats()
+    assert st["failed"] == 0 and st["completed"] == len(reqs)
+    for rid, (hk, hv, k, v, desc) in zip(ids, reqs):
+        assert same(hk, k.cpu()), f"host request {rid} keys differ from the alone sort"
+        assert same(hv, None if v is None else v.cpu()), f"host request {rid} payloads differ"
+
+
+def test_host_and_device_requests_mixed(handle):
+    reqs = []
+    for i, n in enumerate(sizes_mix(5, 30)):
+        kind = KINDS[i % len(KINDS)]
+        k, v = make_request(handle, kind, int(n), 5000 + i)
+        k0, v0 = k.clone(), None if v is None else v.clone()
+        if v is None:
+            handle.sort_keys(k0)
+        else:
+            handle.sort_pairs(k0, v0)
+        torch.cuda.synchronize()
+        on_host = i % 2 == 1
+        if on_host:
+            k, v = pinned_copy(k), None if v is None else pinned_copy(v)
+        reqs.append((on_host, k, v, k0, v0))
+    say(f"mixed host/device requests count={len(reqs)}")
+    with g.Server(streams=3) as srv:
+        for on_host, k, v, _, _ in reqs:
+            (srv.submit_host if on_host else srv.submit)(k, v)
+        srv.drain()
+    for on_host, k, v, k0, v0 in reqs:
+        assert same(k if on_host else k.cpu(), k0.cpu())
+        if v is not None:
+            assert same(v if on_host else v.cpu(), v0.cpu())
