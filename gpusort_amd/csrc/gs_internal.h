@@ -37,6 +37,9 @@ struct SortCtrl {
   uint32_t tile_counter[kMaxPasses];
   uint32_t final_sel;
   uint32_t passes_run;
+  // Tile tickets per workgroup class (blockIdx % 8) when a pass hands out
+  // tiles in XCD groups.
+  uint32_t class_counter[kMaxPasses][8];
   uint32_t pad[14];
 };
 

@@ -126,6 +126,7 @@ __global__ __launch_bounds__(256) void radix_scan_kernel(const unsigned long lon
     uint32_t sel = 0, run = 0;
     for (int p = 0; p < kMaxPasses; ++p) {
       ctrl->tile_counter[p] = 0;
+      for (int c = 0; c < 8; ++c) ctrl->class_counter[p][c] = 0;
       if (p >= passes || s_trivial[p]) {
         ctrl->src[p] = kSkipPass;
       } else {
@@ -153,6 +154,9 @@ struct OnesweepArgs {
   S* status;                   // tiles*256 status words of this pass, zeroed
   SortCtrl* ctrl;
   uint32_t tiles;
+  // Tile hand-out: 0 = one ticket counter in launch order; G > 0 = tiles in
+  // groups of G consecutive ones per workgroup class.
+  uint32_t group = 0;
   // Segmented mode only: tiles never straddle segments and the look-back
   // stops at the segment's first tile.
   const uint32_t* tile_seg;   // tile -> segment slot
@@ -167,6 +171,20 @@ struct OnesweepArgs {
   int upper;  // 1: bucket = #splitters < key; 0: #splitters <= key
   int depth;  // levels of the splitter tree: 2^depth - 1 >= nspl
 };
+
+// Tile of the j-th workgroup of class c (blockIdx % 8) to take a ticket.
+// Tiles come in super-groups of 8*G consecutive tiles, class c taking tiles
+// [c*G, c*G + G) of each; the tiles after the last whole super-group go round
+// robin (tile = base + 8*i + c), which matches the number of workgroups of
+// each class in a grid of `tiles` workgroups.  Within a class the map is
+// increasing in j.  Workgroups are dealt to the 8 XCDs round robin by
+// blockIdx, so consecutive tiles run on one XCD and the cache lines shared
+// by the ends of their digit runs meet in that XCD's L2.
+__device__ inline uint32_t class_tile(uint32_t c, uint32_t j, uint32_t G, uint32_t tiles) {
+  const uint32_t full = tiles / (8u * G);
+  if (j < full * G) return (j / G) * 8u * G + c * G + (j % G);
+  return full * 8u * G + (j - full * G) * 8u + c;
+}
 
 // Levels of the smallest perfect search tree that holds nspl splitters.
 inline int tree_depth(uint32_t nspl) {
@@ -263,7 +281,14 @@ __global__ __launch_bounds__(THREADS) void onesweep_kernel(OnesweepArgs<U, V, S>
   const uint32_t lane = tid & 63u;
   const uint32_t wave = tid >> 6;
 
-  if (tid == 0) s_tile = atomicAdd(&a.ctrl->tile_counter[a.pass], 1u);
+  if (tid == 0) {
+    if (a.group) {
+      const uint32_t c = blockIdx.x & 7u;
+      s_tile = class_tile(c, atomicAdd(&a.ctrl->class_counter[a.pass][c], 1u), a.group, a.tiles);
+    } else {
+      s_tile = atomicAdd(&a.ctrl->tile_counter[a.pass], 1u);
+    }
+  }
   for (uint32_t i = tid; i < WAVES * kRadix; i += THREADS) {
     s_wave_hist[i] = 0;
     s_match[i] = 0;
@@ -598,6 +623,27 @@ static Xform<U> xform_of(const XformParams& p) {
   return Xform<U>{(U)p.float_mask, (U)p.sign_flip, (U)p.desc_mask};
 }
 
+// Tiles per XCD group of the single-array pass.  A tile's look-back waits on
+// every lower tile; with grouped hand-out the lower tiles of its super-group
+// may belong to workgroups dispatched up to 8*G positions later, so 8*G must
+// stay well inside the number of resident workgroups (at least one per CU):
+// G is the largest power of two up to 16 with 16*G <= CUs, 0 (one ticket
+// counter) below 16 CUs.
+static uint32_t tile_group() {
+  static int cache[64];
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return 0;
+  if (cache[dev] == 0) {
+    int cus = 0;
+    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess) cus = 0;
+    int g = 0;
+    for (int c = 1; c <= 16; c <<= 1)
+      if (16 * c <= cus) g = c;
+    cache[dev] = g + 1;  // 0 = not yet known
+  }
+  return (uint32_t)(cache[dev] - 1);
+}
+
 template <typename U, int VB, typename S>
 static hipError_t radix_sort_t(U* keys, void* vals_v, uint64_t n, const XformParams& xp, unsigned char* ws,
                                const RadixLayout& L, hipStream_t stream, LaunchTimer* timer) {
@@ -644,6 +690,7 @@ static hipError_t radix_sort_t(U* keys, void* vals_v, uint64_t n, const XformPar
     a.status = status + p * status_per_pass;
     a.ctrl = ctrl;
     a.tiles = (uint32_t)L.tiles;
+    a.group = tile_group();
     if (timer) timer->begin("onesweep_pass", 2 * n * (sizeof(U) + (KV ? sizeof(V) : 0)), stream);
     if (ident)
       hipLaunchKernelGGL((onesweep_kernel<U, V, KV, THREADS, ITEMS, S, true, false, Idx>), dim3((uint32_t)L.tiles),
@@ -757,6 +804,7 @@ __global__ __launch_bounds__(256) void seg_scan_kernel(uint32_t* seg_cnt, int pa
     __syncthreads();
   }
   if (blockIdx.x == 0 && tid < kMaxPasses) ctrl->tile_counter[tid] = 0;
+  if (blockIdx.x == 0 && tid < kMaxPasses * 8) ctrl->class_counter[tid / 8][tid % 8] = 0;
 }
 
 struct SegLayout {
@@ -840,6 +888,7 @@ static hipError_t radix_segments_t(U* keys, void* vals_v, const SegTiles& T, con
     a.seg_range = T.seg_range;
     a.seg_base = cnt;
     a.passes = L.passes;
+    a.group = tile_group();
     if (timer) timer->begin("seg_onesweep_pass", 2 * T.n_keys * (sizeof(U) + (KV ? sizeof(V) : 0)), stream);
     if (ident)
       hipLaunchKernelGGL((onesweep_kernel<U, V, KV, THREADS, ITEMS, uint32_t, true, true, Idx>),
@@ -984,6 +1033,7 @@ static hipError_t multisplit_t(const U* keys, const void* vals, uint64_t n, cons
   a.nspl = nspl;
   a.upper = upper;
   a.depth = depth;
+  a.group = tile_group();
   if (timer) timer->begin("multisplit_pass", 2 * n * (sizeof(U) + (KV ? sizeof(V) : 0)), stream);
   hipLaunchKernelGGL((onesweep_kernel<U, V, KV, THREADS, ITEMS, S, false, false, Idx, true>), dim3((uint32_t)L.tiles),
                      dim3(THREADS), 0, stream, a);
