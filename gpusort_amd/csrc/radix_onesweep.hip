@@ -69,18 +69,38 @@ struct Vec16<uint64_t> {
   __device__ static uint64_t get(const T& v, int i) { return i == 0 ? v.x : v.y; }
 };
 
+// Digit histograms of every pass in one read of the keys.  Each lane adds
+// into its own copy of the counters: copy c = lane % 32 sits in LDS bank c
+// (word = (pass * 128 + digit / 2) * 32 + c), so the 64 adds of a wave fall
+// two per bank whatever the digits, where one shared table takes random bank
+// conflicts (about 5-way) on every add.  Two 16-bit counts share a word
+// (digits 2k and 2k + 1); a count holds the keys of the 2 * 16 lanes of the
+// workgroup that share its copy, at most 32 * kHistKeysPerThread < 2^16.
+constexpr int kHistThreads = 1024;
+constexpr uint64_t kHistKeysPerThread = 1024;
+
 template <typename U>
-__global__ __launch_bounds__(256) void radix_hist_kernel(const U* __restrict__ keys, uint64_t n, Xform<U> xf,
-                                                         int passes, unsigned long long* __restrict__ g_hist,
-                                                         uint4* __restrict__ zero_ptr, uint64_t zero_vec4) {
-  __shared__ uint32_t s_hist[kMaxPasses * kRadix];
+__global__ __launch_bounds__(kHistThreads) void radix_hist_kernel(const U* __restrict__ keys, uint64_t n, Xform<U> xf,
+                                                                  int passes, unsigned long long* __restrict__ g_hist,
+                                                                  uint4* __restrict__ zero_ptr, uint64_t zero_vec4) {
+  constexpr int MAXP = sizeof(U);  // one 8-bit digit per key byte
+  constexpr uint32_t WORDS = MAXP * (kRadix / 2) * 32;
+  __shared__ __attribute__((aligned(16))) uint32_t s_cnt[WORDS];
   const uint32_t tid = threadIdx.x;
-  for (uint32_t i = tid; i < (uint32_t)passes * kRadix; i += 256) s_hist[i] = 0;
-  const uint64_t gtid = (uint64_t)blockIdx.x * 256 + tid;
-  const uint64_t gstride = (uint64_t)gridDim.x * 256;
+  const uint32_t copy = tid & 31u;
+  uint4* s4 = reinterpret_cast<uint4*>(s_cnt);
+  for (uint32_t i = tid; i < (uint32_t)passes * (kRadix / 2) * 32 / 4; i += kHistThreads) s4[i] = make_uint4(0, 0, 0, 0);
+  const uint64_t gtid = (uint64_t)blockIdx.x * kHistThreads + tid;
+  const uint64_t gstride = (uint64_t)gridDim.x * kHistThreads;
   for (uint64_t i = gtid; i < zero_vec4; i += gstride) zero_ptr[i] = make_uint4(0, 0, 0, 0);
   __syncthreads();
 
+  auto count = [&](U u) {
+    for (int p = 0; p < passes; ++p) {
+      const uint32_t d = digit_of(u, p * 8);
+      atomicAdd(&s_cnt[((uint32_t)p * (kRadix / 2) + (d >> 1)) * 32u + copy], 1u << ((d & 1u) << 4));
+    }
+  };
   using VT = typename Vec16<U>::T;
   constexpr int VN = Vec16<U>::N;
   const uint64_t nvec = n / VN;
@@ -88,20 +108,37 @@ __global__ __launch_bounds__(256) void radix_hist_kernel(const U* __restrict__ k
   for (uint64_t vi = gtid; vi < nvec; vi += gstride) {
     const VT v = kv[vi];
 #pragma unroll
-    for (int e = 0; e < VN; ++e) {
-      const U u = to_ordered(Vec16<U>::get(v, e), xf);
-      for (int p = 0; p < passes; ++p) hist_add<U>(s_hist + p * kRadix, digit_of(u, p * 8));
-    }
+    for (int e = 0; e < VN; ++e) count(to_ordered(Vec16<U>::get(v, e), xf));
   }
-  if (blockIdx.x == 0) {
-    for (uint64_t i = nvec * VN + tid; i < n; i += 256) {
-      const U u = to_ordered(keys[i], xf);
-      for (int p = 0; p < passes; ++p) atomicAdd(&s_hist[p * kRadix + digit_of(u, p * 8)], 1u);
-    }
-  }
+  if (blockIdx.x == 0)
+    for (uint64_t i = nvec * VN + tid; i < n; i += kHistThreads) count(to_ordered(keys[i], xf));
   __syncthreads();
-  for (uint32_t i = tid; i < (uint32_t)passes * kRadix; i += 256)
-    if (s_hist[i]) atomicAdd(&g_hist[i], (unsigned long long)s_hist[i]);
+  // One thread per digit pair: the 32 copies summed, read in an order
+  // rotated by thread so that a wave's reads spread over the banks.
+  for (uint32_t w = tid; w < (uint32_t)passes * (kRadix / 2); w += kHistThreads) {
+    uint32_t lo = 0, hi = 0;
+#pragma unroll 8
+    for (uint32_t c = 0; c < 32; ++c) {
+      const uint32_t x = s_cnt[w * 32u + ((c + w) & 31u)];
+      lo += x & 0xFFFFu;
+      hi += x >> 16;
+    }
+    const uint32_t p = w / (kRadix / 2), d = (w % (kRadix / 2)) * 2;
+    if (lo) atomicAdd(&g_hist[p * kRadix + d], (unsigned long long)lo);
+    if (hi) atomicAdd(&g_hist[p * kRadix + d + 1], (unsigned long long)hi);
+  }
+}
+
+// Workgroups of the histogram kernel: at most kHistKeysPerThread keys per
+// thread (the 16-bit count bound), and up to 512 workgroups to spread the
+// read over the chip.
+static uint64_t hist_blocks_for(uint64_t n) {
+  const uint64_t per_block = (uint64_t)kHistThreads * kHistKeysPerThread;
+  uint64_t blocks = (n + per_block - 1) / per_block;
+  const uint64_t spread = (n + kHistThreads - 1) / kHistThreads;
+  const uint64_t floor = spread < 512 ? spread : 512;
+  if (blocks < floor) blocks = floor;
+  return blocks < 1 ? 1 : blocks;
 }
 
 __global__ __launch_bounds__(256) void radix_scan_kernel(const unsigned long long* __restrict__ g_hist,
@@ -667,12 +704,9 @@ static hipError_t radix_sort_t(U* keys, void* vals_v, uint64_t n, const XformPar
   hipLaunchKernelGGL(fill_u32_kernel, dim3(8), dim3(256), 0, stream, reinterpret_cast<uint32_t*>(hist),
                      (uint64_t)(kMaxPasses * kRadix * 2), 0u);
   const uint64_t status_bytes = (uint64_t)L.passes * status_per_pass * sizeof(S);
-  const uint64_t nvec = n / Vec16<U>::N;
-  uint64_t hist_blocks = (nvec + 256 * 8 - 1) / (256 * 8);
-  if (hist_blocks < 1) hist_blocks = 1;
-  if (hist_blocks > 8192) hist_blocks = 8192;  // 8192 x 256 threads: 0.38 ms per GiB of keys vs 0.41 at 2048
+  const uint64_t hist_blocks = hist_blocks_for(n);
   if (timer) timer->begin("radix_hist", n * sizeof(U) + status_bytes, stream);
-  hipLaunchKernelGGL(radix_hist_kernel<U>, dim3((uint32_t)hist_blocks), dim3(256), 0, stream, keys, n, xf,
+  hipLaunchKernelGGL(radix_hist_kernel<U>, dim3((uint32_t)hist_blocks), dim3(kHistThreads), 0, stream, keys, n, xf,
                      L.passes, hist, reinterpret_cast<uint4*>(status), status_bytes / 16);
   if (timer) timer->end(stream);
   hipLaunchKernelGGL(radix_scan_kernel, dim3(1), dim3(256), 0, stream, hist, base, ctrl, L.passes, n);
