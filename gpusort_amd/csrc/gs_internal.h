@@ -171,6 +171,47 @@ hipError_t net_cuts(int key_bytes, int val_bytes, const void* keys, const void* 
                     uint32_t nops, const XformParams& xf, const XformParams* lex_vxf, int increasing,
                     NetCutResult* d_out, hipStream_t stream);
 
+// ---------------------------------------------------------------- k-way merge
+// One destination of a single-pass k-way merge: m <= kKwayMax sorted runs
+// (offsets into the source buffer, in list order: equal elements come from
+// the earlier run first) merged into [out_off, out_off + sum of lengths) of
+// the destination buffer.  samp_base .. nparts are filled by kway_plan.
+constexpr uint32_t kKwayMax = 8;
+struct KwayDest {
+  uint64_t run_off[kKwayMax];
+  uint64_t run_len[kKwayMax];
+  uint64_t out_off;
+  uint32_t m;
+  uint32_t samp_base;  // first sample of this destination
+  uint32_t bnd_base;   // first of its nparts + 1 partition boundaries
+  uint32_t part_base;  // first of its partitions
+  uint32_t nparts;
+  uint32_t pad;
+};
+// Layout of a k-way pass: kway_plan fills each destination's sample,
+// boundary and partition bases, the destination of every sample block (256
+// samples) and each destination's first sample block.
+struct KwayPlan {
+  std::vector<KwayDest> dests;
+  std::vector<uint32_t> block_dest;
+  std::vector<uint32_t> dest_block0;
+  uint32_t nsamples = 0, nbounds = 0, nparts = 0;
+};
+void kway_plan(KwayPlan* plan);
+// Device scratch of a pass (partition boundaries and compact samples).
+size_t kway_scratch_bytes(const KwayPlan& plan, int kb, int vb);
+// Device copies of the plan's tables, and the scratch.
+struct KwayDevice {
+  const KwayDest* dests;
+  const uint32_t* block_dest;
+  const uint32_t* dest_block0;
+  void* scratch;
+};
+// Merges every destination from (sk, sv) into (dk, dv).  lex_vxf: when given
+// (and there is a payload) elements compare as (key, payload) pairs.
+hipError_t kway_merge(int kb, int vb, const void* sk, const void* sv, void* dk, void* dv, const KwayPlan& plan,
+                      const KwayDevice& dev, const XformParams& xf, const XformParams* lex_vxf, hipStream_t stream);
+
 // ---------------------------------------------------------------- splitting
 // Bucket boundaries of sorted runs.  Run r is keys[run_off[r] .. run_off[r+1])
 // and is cut by splitters[spl_first[r] .. spl_first[r] + nspl).  Output:

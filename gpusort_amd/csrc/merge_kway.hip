@@ -1,0 +1,523 @@
+// Single-pass k-way merge (k <= 8) of sorted runs, batched over many
+// destinations per launch.
+//
+// Device counterpart of HykSortHelper::Merge (include/helpers/
+// hyksort_helper.h:307-332), which merges a rank's k received runs as a
+// binary tree of OmpUtils::Merge calls (log2 k passes over the data).  Here
+// every element is read once and written once:
+//
+//   samples    every kSampleStride-th element of every run is a sample; one
+//              thread per sample finds its rank among all samples of its
+//              destination in merged order (a binary search over the sample
+//              positions of each other run).  Every kSamplesPerPart-th sample
+//              is a partition boundary: its exact split of every run (the
+//              number of the run's elements that precede it) is found by a
+//              binary search over the run.
+//   merge      one workgroup per partition stages the partition's piece of
+//              every run in LDS, merges the pieces there as a tree of
+//              merge-path levels (two LDS buffers), and writes the partition
+//              out in merged order.
+//
+// Merged order: elements compare by ordered key (with LEX, by (key,
+// payload) pairs, the lexicographic order of std::pair under the lifted
+// comparator, include/common.h:156-164); equal elements keep the order of
+// their runs in the list, then their order within the run, which is what the
+// reference's tree of std::merge calls (earlier run first) produces.
+//
+// Partition size bound: a partition holds kSamplesPerPart samples, and the
+// piece of run i in it holds c_i of them, so the piece is shorter than
+// (c_i + 1) * kSampleStride; over m runs a partition has fewer than
+// (kSamplesPerPart + m) * kSampleStride elements, which is the LDS tile.
+#include "gs_device.h"
+#include "gs_internal.h"
+
+namespace gs {
+
+namespace {
+
+constexpr uint32_t kSampleStride = 128;
+constexpr uint32_t kSamplesPerPart = 16;
+constexpr uint32_t kKwayCap = (kSamplesPerPart + kKwayMax) * kSampleStride;
+constexpr int kKwayThreads = 1024;
+constexpr int kKwayItems = kKwayCap / kKwayThreads;
+static_assert(kKwayItems * kKwayThreads == (int)kKwayCap, "tile covers the cap");
+
+// Element of the merged order: ordered key and, with LEX, ordered payload.
+template <typename U, typename V, bool LEX>
+struct KElem {
+  U k;
+  V v;
+  __device__ bool lt(const KElem& o) const {
+    if constexpr (LEX) return k < o.k || (k == o.k && v < o.v);
+    return k < o.k;
+  }
+  __device__ bool le(const KElem& o) const {
+    if constexpr (LEX) return k < o.k || (k == o.k && v <= o.v);
+    return k <= o.k;
+  }
+};
+
+// Does y, an element of run ry, precede x, an element of a different run
+// rx, in merged order?
+template <typename E>
+__device__ inline bool precedes(const E& y, uint32_t ry, const E& x, uint32_t rx) {
+  return ry < rx ? y.le(x) : y.lt(x);
+}
+
+// Number of t in [0, n) with pred(t) true; pred is true on a prefix.
+template <typename P>
+__device__ inline uint64_t count_prefix(uint64_t n, P pred) {
+  uint64_t lo = 0, hi = n;
+  while (lo < hi) {
+    const uint64_t mid = (lo + hi) >> 1;
+    if (pred(mid))
+      lo = mid + 1;
+    else
+      hi = mid;
+  }
+  return lo;
+}
+
+// Largest d < n with field(d) <= g (field is non-decreasing, field(0) = 0).
+template <typename F>
+__device__ inline uint32_t find_dest(uint32_t n, uint32_t g, F field) {
+  uint32_t lo = 0, hi = n;
+  while (hi - lo > 1) {
+    const uint32_t mid = (lo + hi) >> 1;
+    if (field(mid) <= g)
+      lo = mid;
+    else
+      hi = mid;
+  }
+  return lo;
+}
+
+// Boundaries 0 (nothing taken) and nparts (every run whole) of each
+// destination.
+__global__ void kway_bounds_init_kernel(const KwayDest* __restrict__ dests, uint32_t ndests,
+                                        uint64_t* __restrict__ bounds) {
+  const uint32_t d = blockIdx.x * blockDim.x + threadIdx.x;
+  if (d >= ndests) return;
+  const KwayDest& D = dests[d];
+  const uint32_t nparts = dests[d].nparts;
+  for (uint32_t i = 0; i < kKwayMax; ++i) {
+    bounds[(uint64_t)D.bnd_base * kKwayMax + i] = 0;
+    bounds[((uint64_t)D.bnd_base + nparts) * kKwayMax + i] = i < D.m ? D.run_len[i] : 0;
+  }
+}
+
+// Sample g (run i, sample t of its destination) of every destination:
+// its element's ordered key and payload, stored compactly at index g.
+template <typename U, typename V, bool LEX>
+__global__ __launch_bounds__(256) void kway_gather_kernel(const U* __restrict__ keys, const V* __restrict__ vals,
+                                                          const KwayDest* __restrict__ dests, uint32_t ndests,
+                                                          uint32_t nsamples, Xform<U> xf, Xform<V> vxf,
+                                                          U* __restrict__ samp_k, V* __restrict__ samp_v) {
+  const uint32_t g = blockIdx.x * 256 + threadIdx.x;
+  if (g >= nsamples) return;
+  const uint32_t d = find_dest(ndests, g, [&](uint32_t x) { return dests[x].samp_base; });
+  const KwayDest& D = dests[d];
+  uint64_t t = g - D.samp_base;
+  uint32_t i = 0;
+  while (i + 1 < D.m && t >= D.run_len[i] / kSampleStride) {
+    t -= D.run_len[i] / kSampleStride;
+    ++i;
+  }
+  const uint64_t a = D.run_off[i] + (t + 1) * kSampleStride - 1;
+  samp_k[g] = to_ordered(keys[a], xf);
+  if constexpr (LEX) samp_v[g] = to_ordered(vals[a], vxf);
+}
+
+// Rank of every sample among its destination's samples in merged order
+// (binary searches over the compact samples of the other runs, in lockstep).
+// Block b serves destination block_dest[b], samples samp_base + (b -
+// first block of that destination) * 256 + threadIdx.x.  Every
+// kSamplesPerPart-th sample is a partition boundary: it records its run and
+// position, and for every other run the number of that run's samples that
+// precede it (refined to an element count by kway_refine_kernel).
+template <typename U, typename V, bool LEX>
+__global__ __launch_bounds__(256) void kway_sample_kernel(const KwayDest* __restrict__ dests,
+                                                          const uint32_t* __restrict__ block_dest,
+                                                          const uint32_t* __restrict__ dest_block0,
+                                                          const U* __restrict__ samp_k,
+                                                          const V* __restrict__ samp_v,
+                                                          uint64_t* __restrict__ bounds,
+                                                          uint32_t* __restrict__ bound_run) {
+  using E = KElem<U, V, LEX>;
+  const uint32_t d = block_dest[blockIdx.x];
+  const KwayDest& D = dests[d];
+  const uint32_t m = D.m;
+  uint32_t sfirst[kKwayMax + 1];  // first compact sample of every run (relative)
+  uint32_t acc = 0;
+#pragma unroll
+  for (uint32_t r = 0; r < kKwayMax; ++r) {
+    sfirst[r] = acc;
+    acc += r < m ? (uint32_t)(D.run_len[r] / kSampleStride) : 0u;
+  }
+  sfirst[kKwayMax] = acc;
+  const uint32_t g = (blockIdx.x - dest_block0[d]) * 256 + threadIdx.x;  // sample within the destination
+  if (g >= acc) return;
+  uint32_t i = 0;
+#pragma unroll
+  for (uint32_t r = 1; r < kKwayMax; ++r) i += (r < m && g >= sfirst[r]) ? 1u : 0u;
+  const uint32_t t = g - sfirst[i];
+  const U* sk = samp_k + D.samp_base;
+  const V* sv = samp_v + D.samp_base;
+  const E x{sk[g], LEX ? sv[g] : V(0)};
+  uint32_t lo[kKwayMax], hi[kKwayMax];
+#pragma unroll
+  for (uint32_t r = 0; r < kKwayMax; ++r) {
+    lo[r] = sfirst[r];
+    hi[r] = (r < m && r != i) ? sfirst[r + 1] : sfirst[r];
+  }
+  for (;;) {
+    bool more = false;
+    E y[kKwayMax];
+    uint32_t mid[kKwayMax];
+#pragma unroll
+    for (uint32_t r = 0; r < kKwayMax; ++r) {
+      mid[r] = (lo[r] + hi[r]) >> 1;
+      if (lo[r] < hi[r]) y[r] = E{sk[mid[r]], LEX ? sv[mid[r]] : V(0)};
+    }
+#pragma unroll
+    for (uint32_t r = 0; r < kKwayMax; ++r) {
+      if (lo[r] < hi[r]) {
+        if (precedes(y[r], r, x, i))
+          lo[r] = mid[r] + 1;
+        else
+          hi[r] = mid[r];
+        more |= lo[r] < hi[r];
+      }
+    }
+    if (!more) break;
+  }
+  uint32_t rank = t;
+#pragma unroll
+  for (uint32_t r = 0; r < kKwayMax; ++r) rank += lo[r] - sfirst[r];
+  if ((rank + 1) % kSamplesPerPart != 0) return;
+  const uint64_t b = (uint64_t)D.bnd_base + (rank + 1) / kSamplesPerPart;  // 1 <= local b < nparts
+  uint64_t* out = bounds + b * kKwayMax;
+#pragma unroll
+  for (uint32_t r = 0; r < kKwayMax; ++r)
+    out[r] = (r == i) ? (uint64_t)(t + 1) * kSampleStride - 1 : (uint64_t)(lo[r] - sfirst[r]);
+  bound_run[b] = i;
+}
+
+// Exact split of every interior boundary: its element x (run i, position j)
+// is preceded by `c` samples of run r and not by the next, so the number of
+// run r's elements preceding x lies in [c * stride, (c + 1) * stride - 1];
+// binary searches over that window (all runs in lockstep).
+template <typename U, typename V, bool LEX>
+__global__ __launch_bounds__(256) void kway_refine_kernel(const U* __restrict__ keys, const V* __restrict__ vals,
+                                                          const KwayDest* __restrict__ dests, uint32_t ndests,
+                                                          uint32_t nbounds, Xform<U> xf, Xform<V> vxf,
+                                                          uint64_t* __restrict__ bounds,
+                                                          const uint32_t* __restrict__ bound_run) {
+  using E = KElem<U, V, LEX>;
+  const uint32_t gb = blockIdx.x * 256 + threadIdx.x;
+  if (gb >= nbounds) return;
+  const uint32_t d = find_dest(ndests, gb, [&](uint32_t x) { return dests[x].bnd_base; });
+  const KwayDest& D = dests[d];
+  const uint32_t b = gb - D.bnd_base;
+  if (b == 0 || b >= D.nparts) return;  // first and last boundaries are fixed
+  const uint32_t m = D.m;
+  const uint32_t i = bound_run[gb];
+  uint64_t* out = bounds + (uint64_t)gb * kKwayMax;
+  auto elem = [&](uint32_t r, uint64_t pos) {
+    const uint64_t a = D.run_off[r] + pos;
+    return E{to_ordered(keys[a], xf), LEX ? to_ordered(vals[a], vxf) : V(0)};
+  };
+  const E x = elem(i, out[i]);
+  uint64_t lo[kKwayMax], hi[kKwayMax];
+#pragma unroll
+  for (uint32_t r = 0; r < kKwayMax; ++r) {
+    if (r < m && r != i) {
+      const uint64_t c = out[r];
+      const uint64_t len = D.run_len[r];
+      const uint64_t top = (c + 1) * kSampleStride - 1;
+      lo[r] = c * kSampleStride;
+      hi[r] = top < len ? top : len;
+    } else {
+      lo[r] = hi[r] = 0;
+    }
+  }
+  for (;;) {
+    bool more = false;
+    E y[kKwayMax];
+    uint64_t mid[kKwayMax];
+#pragma unroll
+    for (uint32_t r = 0; r < kKwayMax; ++r) {
+      mid[r] = (lo[r] + hi[r]) >> 1;
+      if (lo[r] < hi[r]) y[r] = elem(r, mid[r]);
+    }
+#pragma unroll
+    for (uint32_t r = 0; r < kKwayMax; ++r) {
+      if (lo[r] < hi[r]) {
+        if (precedes(y[r], r, x, i))
+          lo[r] = mid[r] + 1;
+        else
+          hi[r] = mid[r];
+        more |= lo[r] < hi[r];
+      }
+    }
+    if (!more) break;
+  }
+#pragma unroll
+  for (uint32_t r = 0; r < kKwayMax; ++r)
+    if (r < m && r != i) out[r] = lo[r];
+}
+
+// Slots of piece group [r0, r1) of a partition: pieces r0..r1-1 (clamped
+// to m) are contiguous in the staging buffer.
+__device__ inline uint32_t group_start(const uint32_t* s_start, uint32_t r, uint32_t m) {
+  return s_start[r < m ? r : m];
+}
+
+template <typename U, typename V, bool KV, bool LEX>
+__global__ __launch_bounds__(kKwayThreads) void kway_merge_kernel(const U* __restrict__ sk, const V* __restrict__ sv,
+                                                                  U* __restrict__ dk, V* __restrict__ dv,
+                                                                  const KwayDest* __restrict__ dests,
+                                                                  uint32_t ndests,
+                                                                  const uint64_t* __restrict__ bounds, Xform<U> xf,
+                                                                  Xform<V> vxf) {
+  using E = KElem<U, V, LEX>;
+  // Elements (ordered key, payload: its ordered image with LEX) of the
+  // groups merged so far, ping-pong between two levels.
+  __shared__ U s_k[2][kKwayCap];
+  __shared__ V s_v[2][KV ? kKwayCap : 1];
+  __shared__ uint64_t s_src[kKwayMax];  // source element of each piece's slot 0
+  __shared__ uint32_t s_start[kKwayMax + 1];
+  __shared__ uint64_t s_out;
+  __shared__ uint32_t s_m;
+
+  const uint32_t tid = threadIdx.x;
+  if (tid == 0) {
+    const uint32_t d = find_dest(ndests, blockIdx.x, [&](uint32_t x) { return dests[x].part_base; });
+    const KwayDest& D = dests[d];
+    const uint64_t q = (uint64_t)D.bnd_base + (blockIdx.x - D.part_base);
+    uint32_t e = 0;
+    uint64_t out = D.out_off;
+    for (uint32_t r = 0; r < kKwayMax; ++r) {
+      const uint64_t lo = bounds[q * kKwayMax + r];
+      const uint64_t hi = bounds[(q + 1) * kKwayMax + r];
+      const uint32_t len = r < D.m ? (uint32_t)(hi - lo) : 0u;
+      s_src[r] = D.run_off[r] + lo - e;  // slot e0 + x of piece r reads s_src[r] + e0 + x
+      s_start[r] = e;
+      e += len;
+      out += r < D.m ? lo : 0;
+    }
+    s_start[kKwayMax] = e;
+    s_out = out;
+    s_m = D.m;
+  }
+  __syncthreads();
+  const uint32_t total = s_start[kKwayMax];
+  const uint32_t m = s_m;
+  const uint64_t out_base = s_out;
+
+  // Stage the pieces in list order (coalesced within each piece; every
+  // thread's loads are issued before any is used).
+  U lk[kKwayItems];
+  V lv[KV ? kKwayItems : 1];
+#pragma unroll
+  for (int u = 0; u < kKwayItems; ++u) {
+    const uint32_t e = tid + (uint32_t)u * kKwayThreads;
+    if (e < total) {
+      uint32_t r = 0;
+#pragma unroll
+      for (uint32_t c = 1; c < kKwayMax; ++c) r += e >= s_start[c] ? 1u : 0u;
+      const uint64_t a = s_src[r] + e;
+      lk[u] = sk[a];
+      if constexpr (KV) lv[u] = sv[a];
+    }
+  }
+#pragma unroll
+  for (int u = 0; u < kKwayItems; ++u) {
+    const uint32_t e = tid + (uint32_t)u * kKwayThreads;
+    if (e < total) {
+      s_k[0][e] = to_ordered(lk[u], xf);
+      if constexpr (KV) s_v[0][e] = LEX ? to_ordered(lv[u], vxf) : lv[u];
+    }
+  }
+  __syncthreads();
+
+  // Merge tree over the pieces: at level L group j (pieces [j*2^L,
+  // (j+1)*2^L)) is the merge of its two halves, the earlier half winning
+  // ties.  Each thread writes kKwayItems consecutive positions of the level's
+  // output, one merge-path search per group its range touches.
+  int cur = 0;
+  for (uint32_t width = 1; width < m; width <<= 1) {
+    const U* ck = s_k[cur];
+    const V* cv = s_v[cur];
+    U* nk = s_k[cur ^ 1];
+    V* nv = s_v[cur ^ 1];
+    auto elem = [&](uint32_t slot) { return E{ck[slot], LEX ? cv[slot] : V(0)}; };
+    uint32_t o = tid * kKwayItems;
+    const uint32_t o_end = (o + kKwayItems < total) ? o + kKwayItems : total;
+    while (o < o_end) {
+      uint32_t g0 = 0;
+      while (group_start(s_start, g0 + 2 * width, m) <= o) g0 += 2 * width;
+      const uint32_t a0 = group_start(s_start, g0, m);
+      const uint32_t b0 = group_start(s_start, g0 + width, m);
+      const uint32_t b1 = group_start(s_start, g0 + 2 * width, m);
+      const uint32_t na = b0 - a0, nb = b1 - b0;
+      const uint32_t diag = o - a0;
+      uint32_t lo = diag > nb ? diag - nb : 0;
+      uint32_t hi = diag < na ? diag : na;
+      while (lo < hi) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (elem(a0 + mid).le(elem(b0 + diag - 1 - mid)))
+          lo = mid + 1;
+        else
+          hi = mid;
+      }
+      uint32_t ia = a0 + lo, ib = b0 + (diag - lo);
+      const uint32_t seg_end = o_end < b1 ? o_end : b1;
+      U ka = ia < b0 ? ck[ia] : U(0), kb = ib < b1 ? ck[ib] : U(0);
+      V va = (KV && ia < b0) ? cv[ia] : V(0), vb = (KV && ib < b1) ? cv[ib] : V(0);
+      for (; o < seg_end; ++o) {
+        const bool take_a = ib >= b1 || (ia < b0 && E{ka, LEX ? va : V(0)}.le(E{kb, LEX ? vb : V(0)}));
+        if (take_a) {
+          nk[o] = ka;
+          if constexpr (KV) nv[o] = va;
+          ++ia;
+          if (ia < b0) {
+            ka = ck[ia];
+            if constexpr (KV) va = cv[ia];
+          }
+        } else {
+          nk[o] = kb;
+          if constexpr (KV) nv[o] = vb;
+          ++ib;
+          if (ib < b1) {
+            kb = ck[ib];
+            if constexpr (KV) vb = cv[ib];
+          }
+        }
+      }
+    }
+    __syncthreads();
+    cur ^= 1;
+  }
+  for (uint32_t o = tid; o < total; o += kKwayThreads) {
+    dk[out_base + o] = from_ordered(s_k[cur][o], xf);
+    if constexpr (KV) dv[out_base + o] = LEX ? from_ordered(s_v[cur][o], vxf) : s_v[cur][o];
+  }
+}
+
+template <int VB>
+struct KValT {
+  using T = uint32_t;
+};
+template <>
+struct KValT<8> {
+  using T = uint64_t;
+};
+
+// Device scratch layout: boundaries, boundary runs, compact sample keys and
+// payloads.
+struct KwayScratch {
+  size_t bounds, bound_run, samp_k, samp_v, total;
+};
+
+KwayScratch kway_scratch(const KwayPlan& P, int kb, int vb) {
+  KwayScratch L{};
+  size_t o = 0;
+  L.bounds = o;
+  o = align_up(o + (size_t)P.nbounds * kKwayMax * sizeof(uint64_t), 256);
+  L.bound_run = o;
+  o = align_up(o + (size_t)P.nbounds * sizeof(uint32_t), 256);
+  L.samp_k = o;
+  o = align_up(o + (size_t)P.nsamples * kb, 256);
+  L.samp_v = o;
+  o = align_up(o + (size_t)P.nsamples * (vb ? vb : 4), 256);
+  L.total = o;
+  return L;
+}
+
+template <typename U, int VB, bool LEX>
+hipError_t kway_launch(const void* sk, const void* sv, void* dk, void* dv, const KwayPlan& P, const KwayDevice& dev,
+                       const XformParams& xp, const XformParams& vp, hipStream_t stream) {
+  using V = typename KValT<VB>::T;
+  constexpr bool KV = VB != 0;
+  const Xform<U> xf{(U)xp.float_mask, (U)xp.sign_flip, (U)xp.desc_mask};
+  const Xform<V> vxf{(V)vp.float_mask, (V)vp.sign_flip, (V)vp.desc_mask};
+  const KwayScratch L = kway_scratch(P, sizeof(U), VB);
+  char* ws = reinterpret_cast<char*>(dev.scratch);
+  uint64_t* bounds = reinterpret_cast<uint64_t*>(ws + L.bounds);
+  uint32_t* bound_run = reinterpret_cast<uint32_t*>(ws + L.bound_run);
+  U* samp_k = reinterpret_cast<U*>(ws + L.samp_k);
+  V* samp_v = reinterpret_cast<V*>(ws + L.samp_v);
+  const uint32_t ndests = (uint32_t)P.dests.size();
+  hipLaunchKernelGGL(kway_bounds_init_kernel, dim3((ndests + 255) / 256), dim3(256), 0, stream, dev.dests, ndests,
+                     bounds);
+  if (P.nsamples) {
+    hipLaunchKernelGGL((kway_gather_kernel<U, V, LEX>), dim3((P.nsamples + 255) / 256), dim3(256), 0, stream,
+                       reinterpret_cast<const U*>(sk), reinterpret_cast<const V*>(sv), dev.dests, ndests, P.nsamples,
+                       xf, vxf, samp_k, samp_v);
+    hipLaunchKernelGGL((kway_sample_kernel<U, V, LEX>), dim3((uint32_t)P.block_dest.size()), dim3(256), 0, stream,
+                       dev.dests, dev.block_dest, dev.dest_block0, samp_k, samp_v, bounds, bound_run);
+    hipLaunchKernelGGL((kway_refine_kernel<U, V, LEX>), dim3((P.nbounds + 255) / 256), dim3(256), 0, stream,
+                       reinterpret_cast<const U*>(sk), reinterpret_cast<const V*>(sv), dev.dests, ndests, P.nbounds,
+                       xf, vxf, bounds, bound_run);
+  }
+  hipLaunchKernelGGL((kway_merge_kernel<U, V, KV, LEX>), dim3(P.nparts), dim3(kKwayThreads), 0, stream,
+                     reinterpret_cast<const U*>(sk), reinterpret_cast<const V*>(sv), reinterpret_cast<U*>(dk),
+                     reinterpret_cast<V*>(dv), dev.dests, ndests, bounds, xf, vxf);
+  return hipGetLastError();
+}
+
+}  // namespace
+
+void kway_plan(KwayPlan* P) {
+  uint32_t samp = 0, bnd = 0, part = 0;
+  P->block_dest.clear();
+  P->dest_block0.assign(P->dests.size(), 0);
+  for (size_t d = 0; d < P->dests.size(); ++d) {
+    KwayDest& D = P->dests[d];
+    uint64_t ns = 0;
+    for (uint32_t r = 0; r < D.m; ++r) ns += D.run_len[r] / kSampleStride;
+    for (uint32_t r = D.m; r < kKwayMax; ++r) {
+      D.run_off[r] = 0;
+      D.run_len[r] = 0;
+    }
+    D.samp_base = samp;
+    D.bnd_base = bnd;
+    D.part_base = part;
+    D.nparts = (uint32_t)(ns / kSamplesPerPart) + 1;
+    P->dest_block0[d] = (uint32_t)P->block_dest.size();
+    for (uint64_t b = 0; b < (ns + 255) / 256; ++b) P->block_dest.push_back((uint32_t)d);
+    samp += (uint32_t)ns;
+    bnd += D.nparts + 1;
+    part += D.nparts;
+  }
+  P->nsamples = samp;
+  P->nbounds = bnd;
+  P->nparts = part;
+}
+
+size_t kway_scratch_bytes(const KwayPlan& P, int kb, int vb) { return kway_scratch(P, kb, vb).total; }
+
+hipError_t kway_merge(int kb, int vb, const void* sk, const void* sv, void* dk, void* dv, const KwayPlan& P,
+                      const KwayDevice& dev, const XformParams& xf, const XformParams* lex_vxf, hipStream_t stream) {
+  if (P.dests.empty() || P.nparts == 0) return hipSuccess;
+  const XformParams vz{0, 0, 0};
+  const bool lex = lex_vxf != nullptr && vb != 0;
+  const XformParams& vp = lex ? *lex_vxf : vz;
+#define GS_KWAY_CASE(UT, VBV)                                                                  \
+  return lex ? kway_launch<UT, VBV, true>(sk, sv, dk, dv, P, dev, xf, vp, stream)              \
+             : kway_launch<UT, VBV, false>(sk, sv, dk, dv, P, dev, xf, vp, stream)
+  if (kb == 4) {
+    if (vb == 0) return kway_launch<uint32_t, 0, false>(sk, sv, dk, dv, P, dev, xf, vp, stream);
+    if (vb == 4) GS_KWAY_CASE(uint32_t, 4);
+    if (vb == 8) GS_KWAY_CASE(uint32_t, 8);
+  } else if (kb == 8) {
+    if (vb == 0) return kway_launch<uint64_t, 0, false>(sk, sv, dk, dv, P, dev, xf, vp, stream);
+    if (vb == 4) GS_KWAY_CASE(uint64_t, 4);
+    if (vb == 8) GS_KWAY_CASE(uint64_t, 8);
+  }
+#undef GS_KWAY_CASE
+  return hipErrorInvalidValue;
+}
+
+}  // namespace gs

@@ -9,10 +9,9 @@
 // and the tile is written back coalesced.  Ties are taken from the first run,
 // as std::merge does, so merging runs in input order is stable.
 //
-// Used for the k-way merge of received buckets
-// (HykSortHelper::Merge, include/helpers/hyksort_helper.h:307-332) as a tree
-// of 2-way levels, and for HyperQuickSort's 2-way merge
-// (include/helpers/hyperquicksort_helper.h:183-202).
+// Used by gs_merge (OmpUtils::Merge's seam) and by the merge-splits of
+// SampleSort's small-input bitonic network; the rank rounds' merges of
+// received buckets use the single-pass k-way merge (merge_kway.hip).
 #include "gs_device.h"
 #include "gs_internal.h"
 

@@ -119,9 +119,11 @@ struct AuxLayout {
   }
 };
 
-gs_status upload(Pipe& P, AuxLayout& L, char** base) {
+// Uploads the layout's blocks; `scratch` more device bytes follow them
+// (at base + L.total), not uploaded.
+gs_status upload(Pipe& P, AuxLayout& L, char** base, size_t scratch = 0) {
   void* ws = nullptr;
-  hipError_t e = P.h->arena.get(Arena::kPipeAux, L.total, P.h->stream, &ws);
+  hipError_t e = P.h->arena.get(Arena::kPipeAux, L.total + scratch, P.h->stream, &ws);
   if (e != hipSuccess) return from_hip(e, "pipeline workspace");
   *base = reinterpret_cast<char*>(ws);
   P.stage.assign(L.total, 0);
@@ -194,10 +196,6 @@ struct Run {
   uint64_t off, len;
 };
 
-// Merges, for every destination, its list of runs (read from the current
-// buffer, in list order, ties to the earlier run) into one run placed at
-// dest_off[d] of the result.  Pairwise merge-path levels ping-pong between
-// the two buffers; the result buffer becomes current.
 // Runs 2-way merge jobs from the current buffer into the other one (one
 // launch for all jobs) and makes that buffer current.
 gs_status run_merge_jobs(Pipe& P, const std::vector<MergeJob>& jobs) {
@@ -225,31 +223,71 @@ gs_status run_merge_jobs(Pipe& P, const std::vector<MergeJob>& jobs) {
   return GS_OK;
 }
 
+// Merges, for every destination, its list of runs (read from the current
+// buffer, in list order, ties to the earlier run) into one run placed at
+// dest_off[d] of the result: one k-way pass for lists of up to kKwayMax
+// runs; each pass ping-pongs between the two buffers and the result buffer
+// becomes current.
 gs_status merge_lists(Pipe& P, std::vector<std::vector<Run>> lists, const std::vector<uint64_t>& dest_off) {
   for (;;) {
-    bool any = false;
-    for (auto& l : lists) any |= l.size() > 1;
-    bool first_level_pending = false;
-    for (size_t d = 0; d < lists.size(); ++d)
-      if (lists[d].size() == 1 && lists[d][0].off != dest_off[d] && lists[d][0].len) first_level_pending = true;
-    if (!any && !first_level_pending) break;
-    // One level: pairs (0,1), (2,3), ...; a lone last run is copied (merged
-    // with an empty run) so every level writes the whole buffer it owns.
-    std::vector<MergeJob> jobs;
+    // Done once every destination holds at most one non-empty run, placed
+    // at its offset.
+    bool need = false;
+    for (size_t d = 0; d < lists.size(); ++d) {
+      size_t live = 0;
+      uint64_t at = dest_off[d];
+      for (const Run& r : lists[d])
+        if (r.len) {
+          ++live;
+          at = r.off;
+        }
+      if (live > 1 || at != dest_off[d]) need = true;
+    }
+    if (!need) break;
+    // One pass over every destination: up to kKwayMax runs merged in one
+    // k-way step (a single run is copied, so the pass writes the whole
+    // buffer it owns); longer lists merge consecutive groups of kKwayMax
+    // runs, in list order, and go round again.
+    KwayPlan plan;
     std::vector<std::vector<Run>> next(lists.size());
     for (size_t d = 0; d < lists.size(); ++d) {
+      std::vector<Run> live;
+      for (const Run& r : lists[d])
+        if (r.len) live.push_back(r);
       uint64_t pos = dest_off[d];
-      const auto& l = lists[d];
-      for (size_t j = 0; j < l.size(); j += 2) {
-        const Run a = l[j];
-        const Run b = (j + 1 < l.size()) ? l[j + 1] : Run{a.off, 0};
-        jobs.push_back(MergeJob{a.off, a.len, b.off, b.len, pos});
-        next[d].push_back(Run{pos, a.len + b.len});
-        pos += a.len + b.len;
+      for (size_t j = 0; j < live.size(); j += kKwayMax) {
+        KwayDest D{};
+        const size_t m = std::min<size_t>(kKwayMax, live.size() - j);
+        uint64_t len = 0;
+        for (size_t r = 0; r < m; ++r) {
+          D.run_off[r] = live[j + r].off;
+          D.run_len[r] = live[j + r].len;
+          len += live[j + r].len;
+        }
+        D.m = (uint32_t)m;
+        D.out_off = pos;
+        plan.dests.push_back(D);
+        next[d].push_back(Run{pos, len});
+        pos += len;
       }
     }
-    gs_status st = run_merge_jobs(P, jobs);
+    kway_plan(&plan);
+    AuxLayout L;
+    const size_t o_d = L.add(plan.dests.data(), plan.dests.size() * sizeof(KwayDest));
+    const size_t o_b = L.add(plan.block_dest.data(), plan.block_dest.size() * sizeof(uint32_t));
+    const size_t o_f = L.add(plan.dest_block0.data(), plan.dest_block0.size() * sizeof(uint32_t));
+    char* base = nullptr;
+    const int vb = P.kv ? P.vb : 0;
+    gs_status st = upload(P, L, &base, kway_scratch_bytes(plan, P.kb, vb));
     if (st != GS_OK) return st;
+    const KwayDevice dev{reinterpret_cast<const KwayDest*>(base + o_d), reinterpret_cast<const uint32_t*>(base + o_b),
+                         reinterpret_cast<const uint32_t*>(base + o_f), base + L.total};
+    const int src = P.cur, dst = P.cur ^ 1;
+    hipError_t e = kway_merge(P.kb, vb, P.kbuf[src], P.kv ? P.vbuf[src] : nullptr, P.kbuf[dst],
+                              P.kv ? P.vbuf[dst] : nullptr, plan, dev, P.kx, P.kv ? &P.vx : nullptr, P.h->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(P.h->stream);
+    if (e != hipSuccess) return from_hip(e, "k-way merge");
+    P.cur ^= 1;
     lists.swap(next);
   }
   return GS_OK;
