@@ -15,8 +15,8 @@
 //                (lower_bound for HykSort / HyperQuickSort, upper_bound for
 //                SampleSort)
 //   exchange     bucket sizes -> new rank offsets (host scan of p*k counts)
-//   merge        merge-path tree over each destination's received runs,
-//                reading them in place in the senders' runs
+//   merge        one k-way pass over each destination's received runs
+//                (merge_kway.hip), reading them in place in the senders' runs
 //
 // Order: with a payload, elements compare as (key, payload) pairs under the
 // lifted comparator (std::less/greater of std::pair, include/common.h:
