@@ -25,6 +25,8 @@ def keys_of(dtype, n, seed, distinct=None):
         r = datagen.draws(seed + 1, distinct)[r % np.uint64(distinct)]
     if d == np.float32:
         return ((r >> np.uint64(40)).astype(np.float64) - 2.0**23).astype(np.float32)
+    if d == np.float64:
+        return (r >> np.uint64(12)).astype(np.float64) - 2.0**51
     if d.itemsize == 4:
         return (r >> np.uint64(32)).astype(np.uint32).view(d)
     return r.view(d)
@@ -120,6 +122,37 @@ def test_vrank_matches_cpu_rule(handle, case):
     if desc:
         order = order[::-1]
     assert_bits_equal(want_k, keys[order], "oracle keys are sorted")
+
+
+LARGE_CASES = [
+    # algorithm, p, k, samples, key dtype, payload, descending, distinct keys; 20K-60K keys per rank,
+    # so every destination's merge spans many sampled partitions
+    ("hyksort", 8, 8, 32, np.uint64, True, True, None),
+    ("samplesort", 12, 0, 0, np.int32, True, False, 7),  # 12 runs per destination: two k-way passes
+    ("hyperquicksort", 4, 0, 32, np.float64, False, False, None),
+    ("hyksort", 16, 4, 32, np.uint32, True, False, 2),
+]
+
+
+@pytest.mark.parametrize("case", LARGE_CASES, ids=lambda c: f"{c[0]}-p{c[1]}-k{c[2]}-{np.dtype(c[4]).name}"
+                         f"{'-kv' if c[5] else ''}{'-desc' if c[6] else ''}{'-d' + str(c[7]) if c[7] else ''}")
+def test_vrank_large_ranks(handle, case):
+    algo, p, k, s, dtype, kv, desc, distinct = case
+    off = rank_offsets(p, p * 17 + 3, 20000, 60000)
+    n = int(off[-1])
+    keys = keys_of(dtype, n, p * 5 + n, distinct=distinct)
+    vals = np.arange(n, dtype=np.uint32) if kv else None
+    say(f"vrank large {case} n={n}")
+    want_k, want_v, want = oracle.vrank(keys, off.astype(np.int64), vals, algorithm=algo, k_way=max(k, 2),
+                                        samples_per_rank=max(s, 1), descending=desc)
+    tk = dev(keys)
+    tv = dev(vals) if kv else None
+    got = handle.vrank_sort(tk, off, tv, algorithm=ALGOS[algo], k_way=max(k, 2), samples_per_rank=max(s, 1),
+                            descending=desc)
+    assert_bits_equal(host(tk), want_k, "keys")
+    if kv:
+        assert_bits_equal(host(tv), want_v, "payloads")
+    assert np.array_equal(got["final_rank_sizes"].astype(np.int64), want["final_rank_sizes"]), "final rank sizes"
 
 
 SMALL_CASES = [
