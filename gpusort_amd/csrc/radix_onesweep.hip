@@ -239,29 +239,40 @@ __device__ inline uint32_t tree_node(uint32_t j, int depth) {
   return (1u << (depth - 1 - tz)) + (r >> (tz + 1));
 }
 
+// The tree is held in `ncopy` (a power of two, up to 32) interleaved copies:
+// node i of copy c at tree[i * ncopy + c], and lane l reads copy l % ncopy.
+// With 32 copies every lane of a wave reads its own bank at every level
+// whatever the path its key takes (one shared tree serialises lanes whose
+// paths hit different nodes of one bank).  Size: 2^depth * ncopy elements.
+constexpr uint32_t kTreeCopies = 32;
+
 // Bucket of an ordered key image: `depth` branch-free steps down the tree;
 // padding nodes hold the all-ones image, so the count is clamped to nspl.
 template <typename U>
-__device__ inline uint32_t tree_bucket(const U* tree, U img, int upper, uint32_t nspl, int depth) {
+__device__ inline uint32_t tree_bucket(const U* tree, U img, int upper, uint32_t nspl, int depth, uint32_t ncopy,
+                                       uint32_t copy) {
   uint32_t i = 1;
 #pragma unroll
   for (int l = 0; l < 8; ++l) {
     if (l >= depth) break;
-    const U t = tree[i];
+    const U t = tree[i * ncopy + copy];
     i = 2 * i + (upper ? (img > t) : (img >= t));
   }
   const uint32_t b = i - (1u << depth);
   return b < nspl ? b : nspl;
 }
 
-// Loads nspl sorted splitters into the tree (image order), padding with the
-// largest image.  Every thread of the block takes part; a barrier must follow.
+// Loads nspl sorted splitters into every copy of the tree (image order),
+// padding with the largest image.  Every thread of the block takes part; a
+// barrier must follow.
 template <typename U>
-__device__ inline void load_tree(U* tree, const U* spl, uint32_t nspl, int depth, const Xform<U>& xf, uint32_t tid,
-                                 uint32_t nthreads) {
+__device__ inline void load_tree(U* tree, const U* spl, uint32_t nspl, int depth, const Xform<U>& xf, uint32_t ncopy,
+                                 uint32_t tid, uint32_t nthreads) {
   const uint32_t nodes = (1u << depth) - 1;
-  for (uint32_t j = tid; j < nodes; j += nthreads)
-    tree[tree_node(j, depth)] = j < nspl ? to_ordered(spl[j], xf) : U(~U(0));
+  for (uint32_t x = tid; x < nodes * ncopy; x += nthreads) {
+    const uint32_t j = x / ncopy, c = x % ncopy;
+    tree[tree_node(j, depth) * ncopy + c] = j < nspl ? to_ordered(spl[j], xf) : U(~U(0));
+  }
 }
 
 // Digit of a key: the ordered image's byte at `shift`.  IDENT is the
@@ -296,8 +307,13 @@ __global__ __launch_bounds__(THREADS) void onesweep_kernel(OnesweepArgs<U, V, S>
   __shared__ uint32_t s_wsum[4];
   __shared__ uint32_t s_tile;
   __shared__ __attribute__((aligned(16))) unsigned char s_buf[TILE * BUF_ELEM];
-  __shared__ U s_tree[SPLIT ? 256 : 1];
   __shared__ unsigned char s_bkt[SPLIT ? TILE : 1];  // bucket of the key in each slot
+  // Multisplit: the splitter tree lives in the tile buffer until the buckets
+  // are found (before ranking; the buffer is written after it).  One copy:
+  // the 18 keys' descents of a thread overlap, and copying the tree per tile
+  // cost more than the deeper levels' bank conflicts.
+  U* const s_tree = reinterpret_cast<U*>(s_buf);
+  constexpr uint32_t tree_ncopy = 1;
 
   uint32_t src;
   if constexpr (SPLIT) {
@@ -330,11 +346,12 @@ __global__ __launch_bounds__(THREADS) void onesweep_kernel(OnesweepArgs<U, V, S>
     s_wave_hist[i] = 0;
     s_match[i] = 0;
   }
-  if constexpr (SPLIT) load_tree<U>(s_tree, a.split_keys, a.nspl, a.depth, a.xf, tid, THREADS);
+  if constexpr (SPLIT) load_tree<U>(s_tree, a.split_keys, a.nspl, a.depth, a.xf, tree_ncopy, tid, THREADS);
   lds_barrier();
   // Digit of a key: its byte at `shift`, or its bucket in multisplit mode.
   auto digit_of_key = [&](U key) -> uint32_t {
-    if constexpr (SPLIT) return tree_bucket<U>(s_tree, to_ordered(key, a.xf), a.upper, a.nspl, a.depth);
+    if constexpr (SPLIT)
+      return tree_bucket<U>(s_tree, to_ordered(key, a.xf), a.upper, a.nspl, a.depth, tree_ncopy, 0u);
     return key_digit<U, IDENT>(key, a.xf, shift);
   };
   const uint32_t tile = s_tile;
@@ -976,24 +993,58 @@ hipError_t radix_sort_segments(int kb, int vb, void* keys, void* vals, const Seg
 // perfect splitter tree in LDS.  Output goes to separate buffers, bucket after bucket,
 // keys of a bucket in input order.
 
+// Bucket counts of a multisplit in one read of the keys: kTreeCopies copies
+// of the splitter tree and of the 256 counters, each lane using the copy of
+// its bank (lane % 32), so neither the tree descent nor the count adds
+// conflict across lanes.
+constexpr int kSplitHistThreads = 1024;
+
 template <typename U>
-__global__ __launch_bounds__(256) void split_hist_kernel(const U* __restrict__ keys, uint64_t n, Xform<U> xf,
-                                                         const U* __restrict__ spl, uint32_t nspl, int upper,
-                                                         int depth, unsigned long long* __restrict__ g_hist,
-                                                         uint4* __restrict__ zero_ptr, uint64_t zero_vec4) {
-  __shared__ uint32_t s_hist[kRadix];
-  __shared__ U s_tree[256];
+__global__ __launch_bounds__(kSplitHistThreads) void split_hist_kernel(
+    const U* __restrict__ keys, uint64_t n, Xform<U> xf, const U* __restrict__ spl, uint32_t nspl, int upper,
+    int depth, unsigned long long* __restrict__ g_hist, uint4* __restrict__ zero_ptr, uint64_t zero_vec4) {
+  __shared__ __attribute__((aligned(16))) uint32_t s_cnt[kRadix * kTreeCopies];  // [bucket][copy]
+  __shared__ U s_tree[kRadix * kTreeCopies];
   const uint32_t tid = threadIdx.x;
-  s_hist[tid] = 0;
-  load_tree<U>(s_tree, spl, nspl, depth, xf, tid, 256);
-  const uint64_t gtid = (uint64_t)blockIdx.x * 256 + tid;
-  const uint64_t gstride = (uint64_t)gridDim.x * 256;
+  const uint32_t copy = tid & (kTreeCopies - 1);
+  uint4* c4 = reinterpret_cast<uint4*>(s_cnt);
+  for (uint32_t i = tid; i < kRadix * kTreeCopies / 4; i += kSplitHistThreads) c4[i] = make_uint4(0, 0, 0, 0);
+  load_tree<U>(s_tree, spl, nspl, depth, xf, kTreeCopies, tid, kSplitHistThreads);
+  const uint64_t gtid = (uint64_t)blockIdx.x * kSplitHistThreads + tid;
+  const uint64_t gstride = (uint64_t)gridDim.x * kSplitHistThreads;
   for (uint64_t i = gtid; i < zero_vec4; i += gstride) zero_ptr[i] = make_uint4(0, 0, 0, 0);
   __syncthreads();
-  for (uint64_t i = gtid; i < n; i += gstride)
-    hist_add<U>(s_hist, tree_bucket<U>(s_tree, to_ordered(keys[i], xf), upper, nspl, depth));
+  auto count = [&](U key) {
+    const uint32_t b = tree_bucket<U>(s_tree, to_ordered(key, xf), upper, nspl, depth, kTreeCopies, copy);
+    atomicAdd(&s_cnt[b * kTreeCopies + copy], 1u);
+  };
+  // 16-byte loads, two in flight per thread.
+  using VT = typename Vec16<U>::T;
+  constexpr int VN = Vec16<U>::N;
+  const uint64_t nvec = n / VN;
+  const VT* kv = reinterpret_cast<const VT*>(keys);
+  uint64_t vi = gtid;
+  for (; vi + gstride < nvec; vi += 2 * gstride) {
+    const VT v0 = kv[vi], v1 = kv[vi + gstride];
+#pragma unroll
+    for (int e = 0; e < VN; ++e) count(Vec16<U>::get(v0, e));
+#pragma unroll
+    for (int e = 0; e < VN; ++e) count(Vec16<U>::get(v1, e));
+  }
+  if (vi < nvec) {
+    const VT v0 = kv[vi];
+#pragma unroll
+    for (int e = 0; e < VN; ++e) count(Vec16<U>::get(v0, e));
+  }
+  if (blockIdx.x == 0)
+    for (uint64_t i = nvec * VN + tid; i < n; i += kSplitHistThreads) count(keys[i]);
   __syncthreads();
-  if (s_hist[tid]) atomicAdd(&g_hist[tid], (unsigned long long)s_hist[tid]);
+  if (tid < kRadix) {
+    uint32_t sum = 0;
+#pragma unroll 8
+    for (uint32_t c = 0; c < kTreeCopies; ++c) sum += s_cnt[tid * kTreeCopies + ((c + tid) & (kTreeCopies - 1))];
+    if (sum) atomicAdd(&g_hist[tid], (unsigned long long)sum);
+  }
 }
 
 struct SplitLayout {
@@ -1041,12 +1092,13 @@ static hipError_t multisplit_t(const U* keys, const void* vals, uint64_t n, cons
   hipLaunchKernelGGL(fill_u32_kernel, dim3(1), dim3(256), 0, stream, reinterpret_cast<uint32_t*>(hist),
                      (uint64_t)(kRadix * 2), 0u);
   const uint64_t status_bytes = L.tiles * kRadix * sizeof(S);
-  uint64_t blocks = (n + 256 * 16 - 1) / (256 * 16);
+  uint64_t blocks = (n + kSplitHistThreads * 16 - 1) / (kSplitHistThreads * 16);
   if (blocks < 1) blocks = 1;
-  if (blocks > 2048) blocks = 2048;
+  if (blocks > 512) blocks = 512;
   if (timer) timer->begin("split_hist", n * sizeof(U) + status_bytes, stream);
   const int depth = tree_depth(nspl);
-  hipLaunchKernelGGL(split_hist_kernel<U>, dim3((uint32_t)blocks), dim3(256), 0, stream, keys, n, xf, d_spl, nspl,
+  hipLaunchKernelGGL(split_hist_kernel<U>, dim3((uint32_t)blocks), dim3(kSplitHistThreads), 0, stream, keys, n, xf,
+                     d_spl, nspl,
                      upper, depth, hist, reinterpret_cast<uint4*>(status), status_bytes / 16);
   if (timer) timer->end(stream);
   hipLaunchKernelGGL(radix_scan_kernel, dim3(1), dim3(256), 0, stream, hist, base, ctrl, 1, n);
