@@ -420,18 +420,22 @@ __global__ __launch_bounds__(THREADS) void onesweep_kernel(OnesweepArgs<U, V, S>
   // 2. stable rank inside the wave.  Lanes holding the same digit are found
   //    through LDS: each lane ORs its lane bit into its digit's mask and reads
   //    the mask back, then the mask is cleared; every matching lane then
-  //    writes the same new digit count, so no lane election is needed.
+  //    writes the same new digit count, so no lane election is needed.  The
+  //    OR and the clear touch 32 bits (one bank) per lane, the read 64.
   uint32_t* wh = s_wave_hist + wave * kRadix;
   unsigned long long* wm = s_match + wave * kRadix;
   uint32_t r[ITEMS];
 #pragma unroll
   for (int i = 0; i < ITEMS; ++i) {
     const uint32_t d = digit_at(i);
-    __hip_atomic_fetch_or(&wm[d], 1ull << lane, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+    // A lane sets and clears only its half-wave's 32-bit word of the mask
+    // (lanes 0-31: low word, 32-63: high word); the read takes both.
+    uint32_t* const half = reinterpret_cast<uint32_t*>(&wm[d]) + (lane >> 5);
+    __hip_atomic_fetch_or(half, 1u << (lane & 31u), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
     __builtin_amdgcn_wave_barrier();
     const uint64_t peers = __hip_atomic_load(&wm[d], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
     __builtin_amdgcn_wave_barrier();
-    __hip_atomic_store(&wm[d], 0ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+    __hip_atomic_store(half, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
     const uint32_t below = count_below(peers);
     const uint32_t old = wh[d];
     __builtin_amdgcn_wave_barrier();
