@@ -425,6 +425,16 @@ __global__ __launch_bounds__(THREADS) void onesweep_kernel(OnesweepArgs<U, V, S>
   uint32_t* wh = s_wave_hist + wave * kRadix;
   unsigned long long* wm = s_match + wave * kRadix;
   uint32_t r[ITEMS];
+  // Waves whose first item's 64 keys share one digit (runs of equal keys
+  // left by the previous pass) rank such items without the mask table: 64
+  // lanes on one mask word serialise.  Other waves keep the branch-free loop
+  // and start ranking as soon as their first keys arrive.
+  bool any_uniform;
+  {
+    const uint32_t d = digit_at(0);
+    any_uniform = ballot(d != (uint32_t)__builtin_amdgcn_readfirstlane(d)) == 0;
+  }
+  if (!any_uniform) {
 #pragma unroll
   for (int i = 0; i < ITEMS; ++i) {
     const uint32_t d = digit_at(i);
@@ -442,6 +452,31 @@ __global__ __launch_bounds__(THREADS) void onesweep_kernel(OnesweepArgs<U, V, S>
     wh[d] = old + (uint32_t)__popcll(peers);
     __builtin_amdgcn_wave_barrier();
     r[i] = old + below;
+  }
+  } else {
+#pragma unroll
+  for (int i = 0; i < ITEMS; ++i) {
+    const uint32_t d = digit_at(i);
+    // A lane sets and clears only its half-wave's 32-bit word of the mask
+    // (lanes 0-31: low word, 32-63: high word); the read takes both.
+    uint64_t peers;
+    if (ballot(d != (uint32_t)__builtin_amdgcn_readfirstlane(d)) == 0) {
+      peers = __builtin_amdgcn_read_exec();  // the whole wave holds one digit
+    } else {
+      uint32_t* const half = reinterpret_cast<uint32_t*>(&wm[d]) + (lane >> 5);
+      __hip_atomic_fetch_or(half, 1u << (lane & 31u), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+      __builtin_amdgcn_wave_barrier();
+      peers = __hip_atomic_load(&wm[d], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+      __builtin_amdgcn_wave_barrier();
+      __hip_atomic_store(half, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+    }
+    const uint32_t below = count_below(peers);
+    const uint32_t old = wh[d];
+    __builtin_amdgcn_wave_barrier();
+    wh[d] = old + (uint32_t)__popcll(peers);
+    __builtin_amdgcn_wave_barrier();
+    r[i] = old + below;
+  }
   }
   lds_barrier();
 
