@@ -425,59 +425,44 @@ __global__ __launch_bounds__(THREADS) void onesweep_kernel(OnesweepArgs<U, V, S>
   uint32_t* wh = s_wave_hist + wave * kRadix;
   unsigned long long* wm = s_match + wave * kRadix;
   uint32_t r[ITEMS];
-  // Waves whose first item's 64 keys share one digit (runs of equal keys
-  // left by the previous pass) rank such items without the mask table: 64
-  // lanes on one mask word serialise.  Other waves keep the branch-free loop
-  // and start ranking as soon as their first keys arrive.
-  bool any_uniform;
-  {
-    const uint32_t d = digit_at(0);
-    any_uniform = ballot(d != (uint32_t)__builtin_amdgcn_readfirstlane(d)) == 0;
-  }
-  if (!any_uniform) {
+  // One item: peers of every lane (its digit's lanes), then the wave's
+  // running count of the digit.  A lane sets and clears only its half-wave's
+  // 32-bit word of the mask (lanes 0-31: low word, 32-63: high word); the
+  // read takes both.  With UNIFORM, an item whose 64 keys share one digit
+  // takes all lanes as peers without the table.
+  auto rank_items = [&](auto uniform_tag) {
+    constexpr bool UNIFORM = decltype(uniform_tag)::value;
 #pragma unroll
-  for (int i = 0; i < ITEMS; ++i) {
-    const uint32_t d = digit_at(i);
-    // A lane sets and clears only its half-wave's 32-bit word of the mask
-    // (lanes 0-31: low word, 32-63: high word); the read takes both.
-    uint32_t* const half = reinterpret_cast<uint32_t*>(&wm[d]) + (lane >> 5);
-    __hip_atomic_fetch_or(half, 1u << (lane & 31u), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-    __builtin_amdgcn_wave_barrier();
-    const uint64_t peers = __hip_atomic_load(&wm[d], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-    __builtin_amdgcn_wave_barrier();
-    __hip_atomic_store(half, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-    const uint32_t below = count_below(peers);
-    const uint32_t old = wh[d];
-    __builtin_amdgcn_wave_barrier();
-    wh[d] = old + (uint32_t)__popcll(peers);
-    __builtin_amdgcn_wave_barrier();
-    r[i] = old + below;
-  }
-  } else {
-#pragma unroll
-  for (int i = 0; i < ITEMS; ++i) {
-    const uint32_t d = digit_at(i);
-    // A lane sets and clears only its half-wave's 32-bit word of the mask
-    // (lanes 0-31: low word, 32-63: high word); the read takes both.
-    uint64_t peers;
-    if (ballot(d != (uint32_t)__builtin_amdgcn_readfirstlane(d)) == 0) {
-      peers = __builtin_amdgcn_read_exec();  // the whole wave holds one digit
-    } else {
-      uint32_t* const half = reinterpret_cast<uint32_t*>(&wm[d]) + (lane >> 5);
-      __hip_atomic_fetch_or(half, 1u << (lane & 31u), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+    for (int i = 0; i < ITEMS; ++i) {
+      const uint32_t d = digit_at(i);
+      uint64_t peers;
+      if (UNIFORM && ballot(d != (uint32_t)__builtin_amdgcn_readfirstlane(d)) == 0) {
+        peers = __builtin_amdgcn_read_exec();
+      } else {
+        uint32_t* const half = reinterpret_cast<uint32_t*>(&wm[d]) + (lane >> 5);
+        __hip_atomic_fetch_or(half, 1u << (lane & 31u), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+        __builtin_amdgcn_wave_barrier();
+        peers = __hip_atomic_load(&wm[d], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+        __builtin_amdgcn_wave_barrier();
+        __hip_atomic_store(half, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+      }
+      const uint32_t below = count_below(peers);
+      const uint32_t old = wh[d];
       __builtin_amdgcn_wave_barrier();
-      peers = __hip_atomic_load(&wm[d], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+      wh[d] = old + (uint32_t)__popcll(peers);
       __builtin_amdgcn_wave_barrier();
-      __hip_atomic_store(half, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+      r[i] = old + below;
     }
-    const uint32_t below = count_below(peers);
-    const uint32_t old = wh[d];
-    __builtin_amdgcn_wave_barrier();
-    wh[d] = old + (uint32_t)__popcll(peers);
-    __builtin_amdgcn_wave_barrier();
-    r[i] = old + below;
-  }
-  }
+  };
+  // Runs of equal keys left by the previous pass give whole waves one digit,
+  // and 64 lanes on one mask word serialise.  A wave whose first item is
+  // such takes the checking loop; others keep the branch-free loop, which
+  // starts as soon as their first keys arrive.
+  const uint32_t d_first = digit_at(0);
+  if (ballot(d_first != (uint32_t)__builtin_amdgcn_readfirstlane(d_first)) == 0)
+    rank_items(std::true_type{});
+  else
+    rank_items(std::false_type{});
   lds_barrier();
 
   // 3. per digit: cross-wave totals (published at once), block scan of the
