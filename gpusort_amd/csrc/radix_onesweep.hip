@@ -301,13 +301,20 @@ __global__ __launch_bounds__(THREADS) void onesweep_kernel(OnesweepArgs<U, V, S>
   // tile-local slot of wave w's first key with digit d.
   __shared__ uint32_t s_wave_hist[WAVES * kRadix];
   // s_match[w][d]: lane mask of wave w's lanes holding digit d for the item
-  // being ranked; every entry is back to 0 after each item.
-  __shared__ unsigned long long s_match[WAVES * kRadix];
+  // being ranked; every entry is back to 0 after each item.  Keys-only
+  // passes (not multisplit) keep it in the tile buffer, free until ranking
+  // ends, which leaves LDS for larger tiles.
+  constexpr bool MATCH_IN_BUF = !KV && !SPLIT;
+  __shared__ unsigned long long s_match_own[MATCH_IN_BUF ? 1 : WAVES * kRadix];
   __shared__ Idx s_gofs[kRadix];
   __shared__ uint32_t s_wsum[4];
   __shared__ uint32_t s_tile;
   __shared__ __attribute__((aligned(16))) unsigned char s_buf[TILE * BUF_ELEM];
   __shared__ unsigned char s_bkt[SPLIT ? TILE : 1];  // bucket of the key in each slot
+  static_assert(!MATCH_IN_BUF || TILE * BUF_ELEM >= WAVES * kRadix * sizeof(unsigned long long),
+                "match table fits the tile buffer");
+  unsigned long long* const s_match =
+      MATCH_IN_BUF ? reinterpret_cast<unsigned long long*>(s_buf) : s_match_own;
   // Multisplit: the splitter tree lives in the tile buffer until the buckets
   // are found (before ranking; the buffer is written after it).  One copy:
   // the 18 keys' descents of a thread overlap, and copying the tree per tile
@@ -590,16 +597,17 @@ __global__ __launch_bounds__(256) void radix_final_copy_kernel(U* __restrict__ k
 
 template <typename U, int VB>
 struct Geometry;
-// Tile sizes: 32-bit keys with no or 32-bit payloads use 1024 threads x 18
-// keys (18K-key tiles: fewer look-back steps per key; 18 keys is the most
-// that fits 128 VGPRs without spills), 64-bit keys with no or 32-bit
+// Tile sizes: 32-bit keys with 32-bit payloads use 1024 threads x 18 keys
+// (18K-key tiles: fewer look-back steps per key; 18 keys is the most that
+// fits 128 VGPRs without spills), keys-only 32-bit keys 1024 x 24 (their
+// match table shares the tile buffer), 64-bit keys with no or 32-bit
 // payloads 1024 x 10 (measured best of 512x12, 512x16, 768x12, 1024x8/10/12),
-// 8-byte payloads 512 x 12.
+// 8-byte payloads 512 x 12.  Multisplit passes use SplitGeometry.
 // LDS holds one tile of the wider of key and payload, reused for the payload
 // scatter.
 template <>
 struct Geometry<uint32_t, 0> {
-  static constexpr int THREADS = 1024, ITEMS = 18;
+  static constexpr int THREADS = 1024, ITEMS = 24;
 };
 template <>
 struct Geometry<uint32_t, 4> {
@@ -643,6 +651,23 @@ static inline uint32_t tile_keys_for(int kb, int vb) {
 }
 
 uint32_t radix_tile_keys(int key_bytes, int val_bytes) { return tile_keys_for(key_bytes, val_bytes); }
+
+// Multisplit passes: as Geometry, except keys-only 32-bit keys at 1024 x 18
+// (the bucket byte per slot and the match table do not fit beside a 24K-key
+// tile).
+template <typename U, int VB>
+struct SplitGeometry {
+  static constexpr int THREADS = Geometry<U, VB>::THREADS, ITEMS = Geometry<U, VB>::ITEMS;
+};
+template <>
+struct SplitGeometry<uint32_t, 0> {
+  static constexpr int THREADS = 1024, ITEMS = 18;
+};
+
+static inline uint32_t split_tile_keys(int kb, int vb) {
+  if (kb == 4 && vb == 0) return SplitGeometry<uint32_t, 0>::THREADS * SplitGeometry<uint32_t, 0>::ITEMS;
+  return tile_keys_for(kb, vb);
+}
 
 // Tile geometries of the segmented sort: 0 = the single-array geometry above,
 // 1 = 256 threads (4K keys for 32-bit keys with 0/4-byte payloads, else 3K),
@@ -1079,7 +1104,7 @@ struct SplitLayout {
 
 static SplitLayout split_layout(int kb, int vb, uint64_t n) {
   SplitLayout L{};
-  L.tiles = (n + tile_keys_for(kb, vb) - 1) / tile_keys_for(kb, vb);
+  L.tiles = (n + split_tile_keys(kb, vb) - 1) / split_tile_keys(kb, vb);
   L.status_word = (n < (1ull << 30)) ? 4 : 8;
   size_t o = 0;
   L.off_ctrl = o;
@@ -1106,8 +1131,8 @@ static hipError_t multisplit_t(const U* keys, const void* vals, uint64_t n, cons
   using V = typename ValT<VB>::T;
   using Idx = typename std::conditional<sizeof(S) == 4, uint32_t, uint64_t>::type;
   constexpr bool KV = VB != 0;
-  constexpr int THREADS = Geometry<U, VB>::THREADS;
-  constexpr int ITEMS = Geometry<U, VB>::ITEMS;
+  constexpr int THREADS = SplitGeometry<U, VB>::THREADS;
+  constexpr int ITEMS = SplitGeometry<U, VB>::ITEMS;
   const Xform<U> xf = xform_of<U>(xp);
   SortCtrl* ctrl = reinterpret_cast<SortCtrl*>(ws + L.off_ctrl);
   unsigned long long* hist = reinterpret_cast<unsigned long long*>(ws + L.off_hist);
