@@ -301,10 +301,10 @@ __global__ __launch_bounds__(THREADS) void onesweep_kernel(OnesweepArgs<U, V, S>
   // tile-local slot of wave w's first key with digit d.
   __shared__ uint32_t s_wave_hist[WAVES * kRadix];
   // s_match[w][d]: lane mask of wave w's lanes holding digit d for the item
-  // being ranked; every entry is back to 0 after each item.  Keys-only
-  // passes (not multisplit) keep it in the tile buffer, free until ranking
-  // ends, which leaves LDS for larger tiles.
-  constexpr bool MATCH_IN_BUF = !KV && !SPLIT;
+  // being ranked; every entry is back to 0 after each item.  Passes other
+  // than multisplit keep it in the tile buffer, free until ranking ends,
+  // which leaves LDS for larger tiles.
+  constexpr bool MATCH_IN_BUF = !SPLIT;
   __shared__ unsigned long long s_match_own[MATCH_IN_BUF ? 1 : WAVES * kRadix];
   __shared__ Idx s_gofs[kRadix];
   __shared__ uint32_t s_wsum[4];
@@ -597,10 +597,10 @@ __global__ __launch_bounds__(256) void radix_final_copy_kernel(U* __restrict__ k
 
 template <typename U, int VB>
 struct Geometry;
-// Tile sizes: 32-bit keys with 32-bit payloads use 1024 threads x 18 keys
-// (18K-key tiles: fewer look-back steps per key; 18 keys is the most that
-// fits 128 VGPRs without spills), keys-only 32-bit keys 1024 x 24 (their
-// match table shares the tile buffer), 64-bit keys with no or 32-bit
+// Tile sizes: 32-bit keys with 32-bit payloads use 1024 threads x 19 keys
+// (19K-key tiles: fewer look-back steps per key; 19 keys is the most that
+// fits 128 VGPRs without spills in every instantiation), keys-only 32-bit
+// keys 1024 x 24 (the match table shares the tile buffer in both), 64-bit keys with no or 32-bit
 // payloads 1024 x 10 (measured best of 512x12, 512x16, 768x12, 1024x8/10/12),
 // 8-byte payloads 512 x 12.  Multisplit passes use SplitGeometry.
 // LDS holds one tile of the wider of key and payload, reused for the payload
@@ -611,7 +611,7 @@ struct Geometry<uint32_t, 0> {
 };
 template <>
 struct Geometry<uint32_t, 4> {
-  static constexpr int THREADS = 1024, ITEMS = 18;
+  static constexpr int THREADS = 1024, ITEMS = 19;
 };
 template <>
 struct Geometry<uint32_t, 8> {
@@ -653,8 +653,8 @@ static inline uint32_t tile_keys_for(int kb, int vb) {
 uint32_t radix_tile_keys(int key_bytes, int val_bytes) { return tile_keys_for(key_bytes, val_bytes); }
 
 // Multisplit passes: as Geometry, except keys-only 32-bit keys at 1024 x 18
-// (the bucket byte per slot and the match table do not fit beside a 24K-key
-// tile).
+// (the bucket byte per slot and its own match table do not fit beside a
+// 24K-key tile).
 template <typename U, int VB>
 struct SplitGeometry {
   static constexpr int THREADS = Geometry<U, VB>::THREADS, ITEMS = Geometry<U, VB>::ITEMS;
