@@ -600,8 +600,8 @@ struct Geometry;
 // Tile sizes: 32-bit keys with 32-bit payloads use 1024 threads x 19 keys
 // (19K-key tiles: fewer look-back steps per key; 19 keys is the most that
 // fits 128 VGPRs without spills in every instantiation), keys-only 32-bit
-// keys 1024 x 24 (the match table shares the tile buffer in both), 64-bit keys with no or 32-bit
-// payloads 1024 x 10 (measured best of 512x12, 512x16, 768x12, 1024x8/10/12),
+// keys 1024 x 24, 64-bit keys with 32-bit payloads 1024 x 12, 64-bit keys
+// alone 1024 x 16 (the match table shares the tile buffer in all of these),
 // 8-byte payloads 512 x 12.  Multisplit passes use SplitGeometry.
 // LDS holds one tile of the wider of key and payload, reused for the payload
 // scatter.
@@ -619,11 +619,11 @@ struct Geometry<uint32_t, 8> {
 };
 template <>
 struct Geometry<uint64_t, 0> {
-  static constexpr int THREADS = 1024, ITEMS = 10;
+  static constexpr int THREADS = 1024, ITEMS = 16;
 };
 template <>
 struct Geometry<uint64_t, 4> {
-  static constexpr int THREADS = 1024, ITEMS = 10;
+  static constexpr int THREADS = 1024, ITEMS = 12;
 };
 template <>
 struct Geometry<uint64_t, 8> {
@@ -653,8 +653,8 @@ static inline uint32_t tile_keys_for(int kb, int vb) {
 uint32_t radix_tile_keys(int key_bytes, int val_bytes) { return tile_keys_for(key_bytes, val_bytes); }
 
 // Multisplit passes: as Geometry, except keys-only 32-bit keys at 1024 x 18
-// (the bucket byte per slot and its own match table do not fit beside a
-// 24K-key tile).
+// and 64-bit keys with no or 32-bit payloads at 1024 x 10 (the bucket byte
+// per slot and its own match table do not fit beside the larger tiles).
 template <typename U, int VB>
 struct SplitGeometry {
   static constexpr int THREADS = Geometry<U, VB>::THREADS, ITEMS = Geometry<U, VB>::ITEMS;
@@ -663,9 +663,19 @@ template <>
 struct SplitGeometry<uint32_t, 0> {
   static constexpr int THREADS = 1024, ITEMS = 18;
 };
+template <>
+struct SplitGeometry<uint64_t, 0> {
+  static constexpr int THREADS = 1024, ITEMS = 10;
+};
+template <>
+struct SplitGeometry<uint64_t, 4> {
+  static constexpr int THREADS = 1024, ITEMS = 10;
+};
 
 static inline uint32_t split_tile_keys(int kb, int vb) {
   if (kb == 4 && vb == 0) return SplitGeometry<uint32_t, 0>::THREADS * SplitGeometry<uint32_t, 0>::ITEMS;
+  if (kb == 8 && vb == 0) return SplitGeometry<uint64_t, 0>::THREADS * SplitGeometry<uint64_t, 0>::ITEMS;
+  if (kb == 8 && vb == 4) return SplitGeometry<uint64_t, 4>::THREADS * SplitGeometry<uint64_t, 4>::ITEMS;
   return tile_keys_for(kb, vb);
 }
 
