@@ -95,10 +95,13 @@ __global__ __launch_bounds__(kHistThreads) void radix_hist_kernel(const U* __res
   for (uint64_t i = gtid; i < zero_vec4; i += gstride) zero_ptr[i] = make_uint4(0, 0, 0, 0);
   __syncthreads();
 
+  // Every key byte is a pass here (passes == MAXP): the pass offsets are
+  // constants of the unrolled loop.
   auto count = [&](U u) {
-    for (int p = 0; p < passes; ++p) {
+#pragma unroll
+    for (int p = 0; p < MAXP; ++p) {
       const uint32_t d = digit_of(u, p * 8);
-      atomicAdd(&s_cnt[((uint32_t)p * (kRadix / 2) + (d >> 1)) * 32u + copy], 1u << ((d & 1u) << 4));
+      atomicAdd(&s_cnt[p * (kRadix / 2) * 32 + (d >> 1) * 32u + copy], 1u << ((d & 1u) << 4));
     }
   };
   using VT = typename Vec16<U>::T;
@@ -781,6 +784,7 @@ static hipError_t radix_sort_t(U* keys, void* vals_v, uint64_t n, const XformPar
                      (uint64_t)(kMaxPasses * kRadix * 2), 0u);
   const uint64_t status_bytes = (uint64_t)L.passes * status_per_pass * sizeof(S);
   const uint64_t hist_blocks = hist_blocks_for(n);
+  // The histogram kernel counts every key byte (L.passes == sizeof(U)).
   if (timer) timer->begin("radix_hist", n * sizeof(U) + status_bytes, stream);
   hipLaunchKernelGGL(radix_hist_kernel<U>, dim3((uint32_t)hist_blocks), dim3(kHistThreads), 0, stream, keys, n, xf,
                      L.passes, hist, reinterpret_cast<uint4*>(status), status_bytes / 16);
