@@ -299,10 +299,18 @@ __global__ __launch_bounds__(THREADS) void onesweep_kernel(OnesweepArgs<U, V, S>
   constexpr S FLAG_PRE = S(2) << FLAG_SHIFT;
   constexpr S VAL_MASK = FLAG_AGG - 1;
   constexpr size_t BUF_ELEM = (sizeof(U) > sizeof(V) || !KV) ? sizeof(U) : sizeof(V);
+  // DUAL: keys and payloads are staged side by side (keys, then payloads),
+  // and written out in one loop; the ranking tables then share the tile
+  // buffer.  Multisplit keeps one buffer reused for the payloads, and so do
+  // geometries where the larger buffer would leave fewer workgroups per CU
+  // than the 16 waves the registers allow (the 160 KB of LDS shared).
+  constexpr bool DUAL =
+      KV && !SPLIT && (1024 / THREADS) * ((size_t)TILE * (sizeof(U) + sizeof(V)) + 2048) <= 160 * 1024;
+  constexpr size_t BUF_BYTES = DUAL ? (size_t)TILE * (sizeof(U) + sizeof(V)) : (size_t)TILE * BUF_ELEM;
 
   // s_wave_hist[w][d]: wave w's count of digit d during ranking, then the
   // tile-local slot of wave w's first key with digit d.
-  __shared__ uint32_t s_wave_hist[WAVES * kRadix];
+  __shared__ uint32_t s_wave_hist_own[DUAL ? 1 : WAVES * kRadix];
   // s_match[w][d]: lane mask of wave w's lanes holding digit d for the item
   // being ranked; every entry is back to 0 after each item.  Passes other
   // than multisplit keep it in the tile buffer, free until ranking ends,
@@ -312,12 +320,14 @@ __global__ __launch_bounds__(THREADS) void onesweep_kernel(OnesweepArgs<U, V, S>
   __shared__ Idx s_gofs[kRadix];
   __shared__ uint32_t s_wsum[4];
   __shared__ uint32_t s_tile;
-  __shared__ __attribute__((aligned(16))) unsigned char s_buf[TILE * BUF_ELEM];
+  __shared__ __attribute__((aligned(16))) unsigned char s_buf[BUF_BYTES];
   __shared__ unsigned char s_bkt[SPLIT ? TILE : 1];  // bucket of the key in each slot
-  static_assert(!MATCH_IN_BUF || TILE * BUF_ELEM >= WAVES * kRadix * sizeof(unsigned long long),
-                "match table fits the tile buffer");
+  static_assert(!MATCH_IN_BUF || BUF_BYTES >= WAVES * kRadix * (sizeof(unsigned long long) + (DUAL ? 4 : 0)),
+                "ranking tables fit the tile buffer");
   unsigned long long* const s_match =
       MATCH_IN_BUF ? reinterpret_cast<unsigned long long*>(s_buf) : s_match_own;
+  uint32_t* const s_wave_hist =
+      DUAL ? reinterpret_cast<uint32_t*>(s_buf + WAVES * kRadix * sizeof(unsigned long long)) : s_wave_hist_own;
   // Multisplit: the splitter tree lives in the tile buffer until the buckets
   // are found (before ranking; the buffer is written after it).  One copy:
   // the 18 keys' descents of a thread overlap, and copying the tree per tile
@@ -508,11 +518,25 @@ __global__ __launch_bounds__(THREADS) void onesweep_kernel(OnesweepArgs<U, V, S>
 
   // 4. local slots; tile sorted by digit in LDS.
   U* s_k = reinterpret_cast<U*>(s_buf);
+  V* s_vd = reinterpret_cast<V*>(s_buf + (size_t)TILE * sizeof(U));
+  if constexpr (DUAL) {
+    // The per-wave offsets sit in the tile buffer: all are read before any
+    // key or payload is staged.
 #pragma unroll
-  for (int i = 0; i < ITEMS; ++i) {
-    r[i] += wh[digit_at(i)];
-    s_k[r[i]] = k[i];
-    if constexpr (SPLIT) s_bkt[r[i]] = (unsigned char)digit_at(i);
+    for (int i = 0; i < ITEMS; ++i) r[i] += wh[digit_at(i)];
+    lds_barrier();
+#pragma unroll
+    for (int i = 0; i < ITEMS; ++i) {
+      s_k[r[i]] = k[i];
+      s_vd[r[i]] = v[i];
+    }
+  } else {
+#pragma unroll
+    for (int i = 0; i < ITEMS; ++i) {
+      r[i] += wh[digit_at(i)];
+      s_k[r[i]] = k[i];
+      if constexpr (SPLIT) s_bkt[r[i]] = (unsigned char)digit_at(i);
+    }
   }
 
   // 5. decoupled look-back, one lane per digit.  Each step reads a window of
@@ -556,7 +580,21 @@ __global__ __launch_bounds__(THREADS) void onesweep_kernel(OnesweepArgs<U, V, S>
   }
   lds_barrier();
 
-  // 6. coalesced scatter of keys in slot order.
+  // 6. coalesced scatter of keys (and, staged beside them, payloads) in slot
+  //    order.
+  if constexpr (DUAL) {
+#pragma unroll
+    for (int j = 0; j < ITEMS; ++j) {
+      const uint32_t p = (uint32_t)j * THREADS + tid;
+      if (p < valid) {
+        const U key = s_k[p];
+        const Idx dst = s_gofs[digit_of_key(key)] + (Idx)p;
+        kout[dst] = key;
+        vout[dst] = s_vd[p];
+      }
+    }
+    return;
+  }
   Idx g[KV ? ITEMS : 1];
 #pragma unroll
   for (int j = 0; j < ITEMS; ++j) {
@@ -569,7 +607,7 @@ __global__ __launch_bounds__(THREADS) void onesweep_kernel(OnesweepArgs<U, V, S>
     }
   }
 
-  // 7. payloads through the same slots.
+  // 7. payloads (multisplit) through the same slots.
   if constexpr (KV) {
     lds_barrier();
     V* s_v = reinterpret_cast<V*>(s_buf);
