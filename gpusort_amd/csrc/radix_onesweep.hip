@@ -638,14 +638,13 @@ __global__ __launch_bounds__(256) void radix_final_copy_kernel(U* __restrict__ k
 
 template <typename U, int VB>
 struct Geometry;
-// Tile sizes: 32-bit keys with 32-bit payloads use 1024 threads x 19 keys
-// (19K-key tiles: fewer look-back steps per key; 19 keys is the most that
-// fits 128 VGPRs without spills in every instantiation), keys-only 32-bit
-// keys 1024 x 24, 64-bit keys with 32-bit payloads 1024 x 12, 64-bit keys
-// alone 1024 x 16 (the match table shares the tile buffer in all of these),
-// 8-byte payloads 512 x 12.  Multisplit passes use SplitGeometry.
-// LDS holds one tile of the wider of key and payload, reused for the payload
-// scatter.
+// Tile sizes, all 1024 threads: 32-bit keys with 32-bit payloads x 19 keys
+// (19K-key tiles: fewer look-back steps per key; a key and a payload per
+// slot then fill the 160 KB of LDS), keys-only 32-bit keys x 24 (the most
+// that fits 128 VGPRs without spills), 64-bit keys alone x 16, 13 when key
+// and payload take 12 bytes, 9 for 64-bit keys with 64-bit payloads (the
+// tile of keys and payloads fits the LDS).  The ranking tables share the
+// tile buffer.  Multisplit passes use SplitGeometry.
 template <>
 struct Geometry<uint32_t, 0> {
   static constexpr int THREADS = 1024, ITEMS = 24;
@@ -656,7 +655,7 @@ struct Geometry<uint32_t, 4> {
 };
 template <>
 struct Geometry<uint32_t, 8> {
-  static constexpr int THREADS = 512, ITEMS = 12;
+  static constexpr int THREADS = 1024, ITEMS = 13;
 };
 template <>
 struct Geometry<uint64_t, 0> {
@@ -664,11 +663,11 @@ struct Geometry<uint64_t, 0> {
 };
 template <>
 struct Geometry<uint64_t, 4> {
-  static constexpr int THREADS = 1024, ITEMS = 12;
+  static constexpr int THREADS = 1024, ITEMS = 13;
 };
 template <>
 struct Geometry<uint64_t, 8> {
-  static constexpr int THREADS = 512, ITEMS = 12;
+  static constexpr int THREADS = 1024, ITEMS = 9;
 };
 
 template <int VB>
@@ -693,9 +692,10 @@ static inline uint32_t tile_keys_for(int kb, int vb) {
 
 uint32_t radix_tile_keys(int key_bytes, int val_bytes) { return tile_keys_for(key_bytes, val_bytes); }
 
-// Multisplit passes: as Geometry, except keys-only 32-bit keys at 1024 x 18
-// and 64-bit keys with no or 32-bit payloads at 1024 x 10 (the bucket byte
-// per slot and its own match table do not fit beside the larger tiles).
+// Multisplit passes: as Geometry, except keys-only 32-bit keys at 1024 x 18,
+// 64-bit keys with no or 32-bit payloads at 1024 x 10 and 32-bit keys with
+// 64-bit payloads at 512 x 12 (the bucket byte per slot, its own match table
+// and one buffer reused for the payloads do not fit beside the larger tiles).
 template <typename U, int VB>
 struct SplitGeometry {
   static constexpr int THREADS = Geometry<U, VB>::THREADS, ITEMS = Geometry<U, VB>::ITEMS;
@@ -703,6 +703,10 @@ struct SplitGeometry {
 template <>
 struct SplitGeometry<uint32_t, 0> {
   static constexpr int THREADS = 1024, ITEMS = 18;
+};
+template <>
+struct SplitGeometry<uint32_t, 8> {
+  static constexpr int THREADS = 512, ITEMS = 12;
 };
 template <>
 struct SplitGeometry<uint64_t, 0> {
@@ -717,6 +721,7 @@ static inline uint32_t split_tile_keys(int kb, int vb) {
   if (kb == 4 && vb == 0) return SplitGeometry<uint32_t, 0>::THREADS * SplitGeometry<uint32_t, 0>::ITEMS;
   if (kb == 8 && vb == 0) return SplitGeometry<uint64_t, 0>::THREADS * SplitGeometry<uint64_t, 0>::ITEMS;
   if (kb == 8 && vb == 4) return SplitGeometry<uint64_t, 4>::THREADS * SplitGeometry<uint64_t, 4>::ITEMS;
+  if (kb == 4 && vb == 8) return SplitGeometry<uint32_t, 8>::THREADS * SplitGeometry<uint32_t, 8>::ITEMS;
   return tile_keys_for(kb, vb);
 }
 
