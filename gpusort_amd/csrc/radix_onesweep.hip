@@ -34,7 +34,7 @@ namespace gs {
 
 constexpr int kRadix = 256;
 // Predecessor status words read per look-back step.
-constexpr int kLookback = 8;
+constexpr int kLookback = 4;
 
 template <typename U>
 __device__ inline void hist_add(uint32_t* h, uint32_t d) {
