@@ -61,19 +61,16 @@ size_t radix_workspace_bytes(int key_bytes, int val_bytes, uint64_t n);
 hipError_t radix_sort(int key_bytes, int val_bytes, void* keys, void* vals, uint64_t n, const XformParams& xf,
                       void* workspace, size_t workspace_bytes, hipStream_t stream, LaunchTimer* timer = nullptr);
 
-// Tile geometry of the onesweep pass for a key/value width pair.
-uint32_t radix_tile_keys(int key_bytes, int val_bytes);
-
 // Segmented onesweep: many segments of one array sorted by one launch per
 // digit.  Segment slot s is [seg_range[2s], seg_range[2s+1]) of the array,
-// shorter than 2^30 keys, cut into tiles of radix_tile_keys() keys numbered
+// shorter than 2^30 keys, cut into tiles of radix_seg_tile_keys() keys numbered
 // segment after segment (seg_first[s] = its first tile, tile_seg[t] = the
 // slot of tile t); slots [0, n_multi) are the ones with more than one tile.
 // All maps are device arrays.  n_total bounds every element offset (the
 // ping-pong buffer in the workspace mirrors the whole array); n_keys is the
 // number of keys in the segments.
 struct SegTiles {
-  int geo;  // tile geometry: 0 = radix_tile_keys(), 1 = the small one (radix_seg_tile_keys)
+  int geo;  // tile geometry (radix_seg_tile_keys): 0 = the large one, 1 = the small one
   const uint32_t* tile_seg;
   const uint32_t* seg_first;
   const uint64_t* seg_range;

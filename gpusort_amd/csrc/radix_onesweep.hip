@@ -462,7 +462,18 @@ __global__ __launch_bounds__(THREADS) void onesweep_kernel(OnesweepArgs<U, V, S>
     constexpr bool UNIFORM = decltype(uniform_tag)::value;
 #pragma unroll
     for (int i = 0; i < ITEMS; ++i) {
-      const uint32_t d = digit_at(i);
+      uint32_t d;
+      if constexpr (SPLIT) {
+        d = digit_at(i);
+      } else {
+        // The empty asm keeps the digit inside this item's iteration: hoisted
+        // ahead of the uniform-wave branch (as the compiler otherwise does),
+        // every digit would wait for every key and hold a register.
+        U kk = k[i];
+        asm volatile("" : "+v"(kk));
+        d = digit_of_key(kk);
+        if (!full && (lofs + i * kWave) >= valid) d = 255u;
+      }
       uint64_t peers;
       if (UNIFORM && ballot(d != (uint32_t)__builtin_amdgcn_readfirstlane(d)) == 0) {
         peers = __builtin_amdgcn_read_exec();
@@ -655,14 +666,14 @@ template <typename U, int VB>
 struct Geometry;
 // Tile sizes, all 1024 threads: 32-bit keys with 32-bit payloads x 19 keys
 // (19K-key tiles: fewer look-back steps per key; a key and a payload per
-// slot then fill the 160 KB of LDS), keys-only 32-bit keys x 24 (the most
+// slot then fill the 160 KB of LDS), keys-only 32-bit keys x 30 (the most
 // that fits 128 VGPRs without spills), 64-bit keys alone x 16, 13 when key
 // and payload take 12 bytes, 9 for 64-bit keys with 64-bit payloads (the
 // tile of keys and payloads fits the LDS).  The ranking tables share the
 // tile buffer.  Multisplit passes use SplitGeometry.
 template <>
 struct Geometry<uint32_t, 0> {
-  static constexpr int THREADS = 1024, ITEMS = 24;
+  static constexpr int THREADS = 1024, ITEMS = 30;
 };
 template <>
 struct Geometry<uint32_t, 4> {
@@ -705,7 +716,6 @@ static inline uint32_t tile_keys_for(int kb, int vb) {
   return Geometry<uint64_t, 8>::THREADS * Geometry<uint64_t, 8>::ITEMS;
 }
 
-uint32_t radix_tile_keys(int key_bytes, int val_bytes) { return tile_keys_for(key_bytes, val_bytes); }
 
 // Multisplit passes: as Geometry, except keys-only 32-bit keys at 1024 x 18,
 // 64-bit keys with no or 32-bit payloads at 1024 x 10 and 32-bit keys with
@@ -740,13 +750,18 @@ static inline uint32_t split_tile_keys(int kb, int vb) {
   return tile_keys_for(kb, vb);
 }
 
-// Tile geometries of the segmented sort: 0 = the single-array geometry above,
+// Tile geometries of the segmented sort: 0 = the single-array geometry above
+// (keys-only 32-bit keys at 1024 x 28: the segmented pass spills at 30),
 // 1 = 256 threads (4K keys for 32-bit keys with 0/4-byte payloads, else 3K),
 // for segments of a few tiles: less of the last tile is padding and up to
-// four workgroups share a CU.
+// four workgroups share a CU.  Geometry 1 has the smallest tiles.
 template <typename U, int VB, int G>
 struct SegGeometry {
   static constexpr int THREADS = Geometry<U, VB>::THREADS, ITEMS = Geometry<U, VB>::ITEMS;
+};
+template <>
+struct SegGeometry<uint32_t, 0, 0> {
+  static constexpr int THREADS = 1024, ITEMS = 28;
 };
 template <typename U, int VB>
 struct SegGeometry<U, VB, 1> {
@@ -754,7 +769,10 @@ struct SegGeometry<U, VB, 1> {
 };
 
 uint32_t radix_seg_tile_keys(int kb, int vb, int geo) {
-  if (geo == 0) return tile_keys_for(kb, vb);
+  if (geo == 0) {
+    if (kb == 4 && vb == 0) return SegGeometry<uint32_t, 0, 0>::THREADS * SegGeometry<uint32_t, 0, 0>::ITEMS;
+    return tile_keys_for(kb, vb);
+  }
   return (kb == 4 && vb <= 4) ? 256 * 16 : 256 * 12;
 }
 

@@ -134,7 +134,8 @@ size_t need_single(gs_dtype kdt, gs_dtype vdt, uint64_t n, bool host) {
 
 size_t need_coalesced(gs_dtype kdt, gs_dtype vdt, uint64_t total, uint64_t count) {
   const int kb = dtype_bytes(kdt), vb = val_bytes_of(vdt);
-  const uint64_t tiles = total / radix_tile_keys(kb, vb) + count;
+  // Upper bound for any tile geometry: the smallest tiles are geometry 1's.
+  const uint64_t tiles = total / radix_seg_tile_keys(kb, vb, 1) + count;
   size_t b = Arena::grown((size_t)total * kb) + Arena::grown(4 * count * sizeof(CopyJob)) +
              Arena::grown(radix_segments_workspace_bytes(kb, vb, total, tiles, (uint32_t)count)) +
              Arena::grown(tiles * 4 + count * 32 + (count + 1) * 8 + 4 * 256);

@@ -17,10 +17,10 @@ pytestmark = pytest.mark.gpu
 # Around every class boundary: LDS tile caps 2048/4096/8192, onesweep tiles
 # 3072/4096 (small), 9216 (64-bit keys with 8-byte payloads), 13312 (12-byte
 # key + payload), 16384 (64-bit keys only), 19456 (32-bit keys with 4-byte
-# payloads) and 24576 (32-bit keys only).
+# payloads) and 28672 (32-bit keys only).
 BOUNDARY_LENGTHS = [0, 1, 2, 3, 2047, 2048, 2049, 4095, 4096, 4097, 6143, 6144, 6145, 8191, 8192, 8193,
                     9215, 9216, 9217, 10239, 10240, 10241, 12288, 12289, 13311, 13312, 13313, 16383, 16384, 16385, 18431, 18432, 18433, 19455, 19456, 19457, 20480, 20481,
-                    24575, 24576, 24577, 26624, 26625, 32768, 36864, 36865, 1, 0, 49153, 50000,
+                    24575, 24576, 24577, 26624, 26625, 28671, 28672, 28673, 30720, 32768, 36864, 36865, 1, 0, 49153, 50000,
                     100000, 262147, 5]
 
 

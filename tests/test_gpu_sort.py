@@ -19,11 +19,11 @@ pytestmark = pytest.mark.gpu
 KEY_DTYPES = [np.uint32, np.int32, np.float32, np.uint64, np.int64, np.float64]
 
 # Sizes around the block-sort cap (8192 / 4096 / 2048) and the onesweep tiles
-# (24576 keys-only u32, 19456 u32 + 32-bit payload, 16384 keys-only u64,
+# (30720 keys-only u32, 19456 u32 + 32-bit payload, 16384 keys-only u64,
 # 13312 with 12-byte key + payload, 9216 u64 + 64-bit payload; 18432, 10240
 # and 6144 for multisplit), plus multi-tile and ragged lengths.
 SIZES = [0, 1, 2, 3, 63, 64, 65, 1000, 2047, 2048, 2049, 4095, 4096, 4097, 6143, 6144, 6145, 8191, 8192, 8193,
-         9215, 9216, 9217, 10239, 10240, 10241, 12287, 12288, 12289, 13311, 13312, 13313, 16383, 16384, 16385, 18431, 18432, 18433, 19455, 19456, 19457, 24575, 24576, 24577, 26625, 49153, 100_003, 1 << 20]
+         9215, 9216, 9217, 10239, 10240, 10241, 12287, 12288, 12289, 13311, 13312, 13313, 16383, 16384, 16385, 18431, 18432, 18433, 19455, 19456, 19457, 24575, 24576, 24577, 26625, 28671, 28672, 28673, 30719, 30720, 30721, 49153, 61441, 100_003, 1 << 20]
 
 
 def make_keys(dtype, n: int, seed: int) -> np.ndarray:
