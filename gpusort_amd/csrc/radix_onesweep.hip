@@ -392,15 +392,12 @@ __global__ __launch_bounds__(THREADS) void onesweep_kernel(OnesweepArgs<U, V, S>
     remain = a.n - tile_base;
   }
   const uint32_t valid = remain < TILE ? (uint32_t)remain : TILE;
-  // Single-array passes compile the rest twice, for full tiles and for a
-  // last, partial one (MODE 1 / 0): with no join between the loads and the
+  // The rest is compiled twice, for full tiles and for a partial last tile
+  // (of the array or of a segment): with no join between the loads and the
   // ranking, each key's wait covers that key's load only, and the payload
-  // loads stay in flight during the ranking.  Segmented passes, whose last
-  // tiles per segment are many, keep one body with the check at run time
-  // (MODE 2; two bodies there spill registers).
-  auto tile_body = [&](auto mode_tag) {
-  constexpr int MODE = decltype(mode_tag)::value;
-  const bool full = MODE == 2 ? valid == TILE : MODE == 1;
+  // loads stay in flight during the ranking.
+  auto tile_body = [&](auto full_tag) {
+  constexpr bool full = decltype(full_tag)::value;
 
   // 1. wave-striped loads: item i of lane l of wave w is key
   //    tile_base + w*64*ITEMS + i*64 + l, so item order within a wave is key order.
@@ -640,12 +637,10 @@ __global__ __launch_bounds__(THREADS) void onesweep_kernel(OnesweepArgs<U, V, S>
     }
   }
   };
-  if constexpr (SEG)
-    tile_body(std::integral_constant<int, 2>{});
-  else if (valid == TILE)
-    tile_body(std::integral_constant<int, 1>{});
+  if (valid == TILE)
+    tile_body(std::true_type{});
   else
-    tile_body(std::integral_constant<int, 0>{});
+    tile_body(std::false_type{});
 }
 
 template <typename U, typename V, bool KV>
