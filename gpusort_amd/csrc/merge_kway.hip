@@ -351,7 +351,6 @@ __global__ __launch_bounds__(kKwayThreads) void kway_merge_kernel(const U* __res
     const V* cv = s_v[cur];
     U* nk = s_k[cur ^ 1];
     V* nv = s_v[cur ^ 1];
-    auto elem = [&](uint32_t slot) { return E{ck[slot], LEX ? cv[slot] : V(0)}; };
     uint32_t o = tid * kKwayItems;
     const uint32_t o_end = (o + kKwayItems < total) ? o + kKwayItems : total;
     while (o < o_end) {
@@ -366,7 +365,15 @@ __global__ __launch_bounds__(kKwayThreads) void kway_merge_kernel(const U* __res
       uint32_t hi = diag < na ? diag : na;
       while (lo < hi) {
         const uint32_t mid = (lo + hi) >> 1;
-        if (elem(a0 + mid).le(elem(b0 + diag - 1 - mid)))
+        const uint32_t pa = a0 + mid, pb = b0 + diag - 1 - mid;
+        const U xa = ck[pa], xb = ck[pb];
+        bool le = xa < xb;
+        if constexpr (LEX) {
+          if (xa == xb) le = cv[pa] <= cv[pb];  // payloads only for equal keys
+        } else {
+          le = xa <= xb;
+        }
+        if (le)
           lo = mid + 1;
         else
           hi = mid;
