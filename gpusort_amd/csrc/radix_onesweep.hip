@@ -15,18 +15,20 @@
 //   copy      only if the result ended in the temporary buffer
 //
 // Onesweep pass, per tile of THREADS*ITEMS keys (tile order from an atomic
-// ticket so every tile's predecessors are already resident):
+// ticket per workgroup class so every tile's predecessors are already
+// resident):
 //   1. coalesced wave-striped loads (lane-consecutive addresses per item)
-//   2. stable ranking inside each wave: ballot match of the 8-bit digit, a
-//      per-wave LDS digit counter updated by the lowest matching lane
+//   2. stable ranking inside each wave: the lanes holding the same 8-bit
+//      digit are found through a per-wave LDS lane-mask table, and a
+//      per-wave LDS digit counter gives each key its rank
 //   3. per-digit scan across waves and digits -> local slot of every key;
 //      the tile's digit counts are published at once (decoupled look-back)
-//   4. keys are written to LDS at their local slot (tile sorted by digit)
+//   4. keys (and payloads, side by side) are written to LDS at their local
+//      slot (tile sorted by digit)
 //   5. 256 look-back lanes accumulate predecessor counts until an inclusive
 //      prefix is found, then publish this tile's inclusive prefix
 //   6. LDS is read back in slot order and written to global memory: lanes
-//      that hold the same digit write consecutive addresses
-//   7. payloads follow the same slots through LDS.
+//      that hold the same digit write consecutive addresses.
 #include "gs_device.h"
 #include "gs_internal.h"
 
