@@ -383,6 +383,60 @@ def run_multisplit(h, g, torch, n: int, nspl: int = 63, steps: int = 5):
             "verified": ok_perm and in_range and stable}
 
 
+def run_global_index(h, g, torch, n: int, steps: int = 5):
+    """Several payload vectors (the variadic HykSort overloads,
+    hyksort_common_vector.inl:552-565, and ParUtils::ArrangeValues,
+    par_utils.h:22-86): uniform u32 keys sorted with an int64 global index as
+    the payload (stable), then three payload vectors (u32, f64, u16) gathered
+    by that index.  Algorithmic bytes of one gather of E-byte elements:
+    n * (8 + 2 * E) (index read, source read, destination write)."""
+    keys0 = torch.empty(n, dtype=torch.uint32, device="cuda")
+    h.generate(keys0, g.GEN_UNIFORM_U32, SEED + 7)
+    gidx0 = torch.empty(n, dtype=torch.uint64, device="cuda")
+    h.generate(gidx0, g.GEN_IOTA_U64, 0)
+    srcs = [torch.arange(n, device="cuda", dtype=torch.int32).view(torch.uint32),
+            torch.arange(n, device="cuda", dtype=torch.float64) * 0.5,
+            (torch.arange(n, device="cuda", dtype=torch.int32) & 0xFFFF).to(torch.int16)]
+    dsts = [torch.empty_like(x) for x in srcs]
+
+    def one(k, gi):
+        h.sort_pairs(k, gi)
+        for src, dst in zip(srcs, dsts):
+            h.gather(src, gi, dst)
+
+    k, gi = keys0.clone(), gidx0.clone()
+    one(k, gi)
+    copies = [(keys0.clone(), gidx0.clone()) for _ in range(steps)]
+    torch.cuda.synchronize()
+    h.profile_begin()
+    t0 = time.perf_counter()
+    for k, gi in copies:
+        one(k, gi)
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    prof = h.profile_end()
+    k, gi = copies[-1]
+    idx = gi.view(torch.int64)
+    kk = as_i64(torch, k)
+    ok = bool((kk[1:] >= kk[:-1]).all()) and bool((as_i64(torch, keys0)[idx] == kk).all())
+    ok = ok and bool(((idx[1:] > idx[:-1]) | (kk[1:] != kk[:-1])).all())
+    same = lambda t: t.view(torch.int32) if t.dtype == torch.uint32 else t  # noqa: E731
+    ok = ok and all(bool((same(dst) == same(src)[idx]).all()) for src, dst in zip(srcs, dsts))
+    gst = prof.get("gather")
+    gather_ms = gst["total_ms"] / steps if gst else None
+    gather_bytes = sum(n * (8 + 2 * x.element_size()) for x in srcs)
+    del copies, keys0, gidx0, srcs, dsts, idx, kk
+    torch.cuda.empty_cache()
+    ms = (t1 - t0) * 1e3 / steps
+    out = {"n": n, "payload_vectors": ["u32", "f64", "u16"], "ms": round(ms, 4),
+           "gkeys_s": round(n / (ms * 1e-3) / 1e9, 3), "verified": ok}
+    if gather_ms:
+        gbs = gather_bytes / (gather_ms * 1e-3) / 1e9
+        out.update({"gathers_ms": round(gather_ms, 4), "gather_GB/s": round(gbs, 1),
+                    "gather_frac": round(gbs / HBM_PEAK_GBS, 4)})
+    return out
+
+
 def run_c1(h, g, torch, with_cpu: bool):
     """Plain single-rank sort: 2^24 uniform u32 keys from seed 1, keys only;
     the GPU result must equal the reference CPU output (OmpUtils::MergeSort,
@@ -719,6 +773,8 @@ def main() -> int:
         extra["single_rank_2^24_u32"] = run_c1(h, g, torch, not args.no_cpu)
         log("[bench] per-rank pipeline: 2^27 u64+u32, p=64, k=8 HykSort (+ CPU re-run)")
         extra["vrank_hyksort_2^27_u64_p64_k8"] = run_c3(h, g, torch, not args.no_cpu)
+        log("[bench] global-index sort 2^27 u32 + gathers of three payload vectors")
+        extra["global_index_u32_2^27_3_payloads"] = run_global_index(h, g, torch, n // 2)
         log("[bench] multisplit 2^28 u32+u32 into 64 buckets")
         extra["multisplit_kv_u32_64_buckets"] = run_multisplit(h, g, torch, n)
         log(f"[bench] batched: {args.arrays} arrays, one call vs one call per array")

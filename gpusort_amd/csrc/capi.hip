@@ -467,7 +467,11 @@ gs_status gs_gather(gs_handle* h, int elem_bytes, const void* d_src, const uint6
   if (n == 0) return GS_OK;
   if (!d_src || !d_idx || !d_dst) return fail(GS_INVALID_ARG, "null pointer");
   DeviceScope ds(h->device);
-  return from_hip(gather_values(elem_bytes, d_src, d_idx, d_dst, n, h->stream), "gather");
+  LaunchTimer* t = h->timer();
+  if (t) t->begin("gather", n * (8 + 2 * (uint64_t)elem_bytes), h->stream);
+  const hipError_t e = gather_values(elem_bytes, d_src, d_idx, d_dst, n, h->stream);
+  if (t) t->end(h->stream);
+  return from_hip(e, "gather");
 }
 
 }  // extern "C"
