@@ -950,20 +950,33 @@ struct SegHistArgs {
 // for single-tile ones); also zeroes the tile's look-back words.
 template <typename U>
 __global__ __launch_bounds__(256) void seg_hist_kernel(SegHistArgs<U> a) {
-  __shared__ uint32_t s_hist[kMaxPasses * kRadix];
+  constexpr int MAXP = sizeof(U);  // one 8-bit digit per key byte (passes == MAXP)
+  __shared__ uint32_t s_hist[MAXP * kRadix];
   const uint32_t tid = threadIdx.x, tile = blockIdx.x;
   const int passes = a.passes;
-  for (uint32_t i = tid; i < (uint32_t)passes * kRadix; i += 256) s_hist[i] = 0;
+  for (uint32_t i = tid; i < (uint32_t)MAXP * kRadix; i += 256) s_hist[i] = 0;
   for (int p = 0; p < passes; ++p) a.status[p * a.status_per_pass + (uint64_t)tile * kRadix + tid] = 0;
   const uint32_t slot = a.tile_seg[tile];
   const uint64_t start = a.seg_range[2 * slot] + (uint64_t)(tile - a.seg_first[slot]) * a.tile_keys;
   const uint64_t seg_end = a.seg_range[2 * slot + 1];
   const uint64_t end = (start + a.tile_keys < seg_end) ? start + a.tile_keys : seg_end;
   __syncthreads();
-  for (uint64_t i = start + tid; i < end; i += 256) {
-    const U u = to_ordered(a.keys[i], a.xf);
-    for (int p = 0; p < passes; ++p) hist_add<U>(s_hist + p * kRadix, digit_of(u, p * 8));
+  auto count = [&](U key) {
+    const U u = to_ordered(key, a.xf);
+#pragma unroll
+    for (int p = 0; p < MAXP; ++p) hist_add<U>(s_hist + p * kRadix, digit_of(u, p * 8));
+  };
+  // Four keys per thread in flight: the loads are issued before any is
+  // counted.
+  uint64_t i = start + tid;
+  for (; i + 3 * 256 < end; i += 4 * 256) {
+    const U k0 = a.keys[i], k1 = a.keys[i + 256], k2 = a.keys[i + 512], k3 = a.keys[i + 768];
+    count(k0);
+    count(k1);
+    count(k2);
+    count(k3);
   }
+  for (; i < end; i += 256) count(a.keys[i]);
   __syncthreads();
   uint32_t* dst = a.seg_cnt + (uint64_t)slot * passes * kRadix;
   if (slot < a.n_multi) {
