@@ -121,29 +121,37 @@ uint32_t block_sort_max(int kb, int vb) {
   return vb == 0 ? BlockCap<uint64_t, 0>::CAP : vb == 4 ? BlockCap<uint64_t, 4>::CAP : BlockCap<uint64_t, 8>::CAP;
 }
 
+// Segments of up to kBlockSortSmall keys take 256-thread workgroups with a
+// kBlockSortSmall-key tile (12-24 KB of LDS instead of up to 48 KB).
 template <typename U, int VB>
 static hipError_t launch_block_sort(void* keys, void* vals, const uint64_t* d_off, const uint32_t* d_ids,
-                                    uint32_t nblocks, uint32_t n, const XformParams& xp, hipStream_t stream) {
+                                    uint32_t nblocks, uint32_t n, const XformParams& xp, hipStream_t stream,
+                                    bool small) {
   using V = typename BValT<VB>::T;
   constexpr bool KV = VB != 0;
   constexpr int CAP = BlockCap<U, VB>::CAP;
+  static_assert(CAP >= (int)kBlockSortSmall, "the small tile is the smaller one");
   const Xform<U> xf{(U)xp.float_mask, (U)xp.sign_flip, (U)xp.desc_mask};
-  hipLaunchKernelGGL((block_sort_kernel<U, V, KV, 512, CAP>), dim3(nblocks), dim3(512), 0, stream,
-                     reinterpret_cast<U*>(keys), reinterpret_cast<V*>(vals), d_off, d_ids, n, xf);
+  if (small)
+    hipLaunchKernelGGL((block_sort_kernel<U, V, KV, 256, kBlockSortSmall>), dim3(nblocks), dim3(256), 0, stream,
+                       reinterpret_cast<U*>(keys), reinterpret_cast<V*>(vals), d_off, d_ids, n, xf);
+  else
+    hipLaunchKernelGGL((block_sort_kernel<U, V, KV, 512, CAP>), dim3(nblocks), dim3(512), 0, stream,
+                       reinterpret_cast<U*>(keys), reinterpret_cast<V*>(vals), d_off, d_ids, n, xf);
   return hipGetLastError();
 }
 
 static hipError_t dispatch_block_sort(int kb, int vb, void* keys, void* vals, const uint64_t* d_off,
                                       const uint32_t* d_ids, uint32_t nblocks, uint32_t n, const XformParams& xf,
-                                      hipStream_t stream) {
+                                      hipStream_t stream, bool small) {
   if (kb == 4) {
-    if (vb == 0) return launch_block_sort<uint32_t, 0>(keys, vals, d_off, d_ids, nblocks, n, xf, stream);
-    if (vb == 4) return launch_block_sort<uint32_t, 4>(keys, vals, d_off, d_ids, nblocks, n, xf, stream);
-    if (vb == 8) return launch_block_sort<uint32_t, 8>(keys, vals, d_off, d_ids, nblocks, n, xf, stream);
+    if (vb == 0) return launch_block_sort<uint32_t, 0>(keys, vals, d_off, d_ids, nblocks, n, xf, stream, small);
+    if (vb == 4) return launch_block_sort<uint32_t, 4>(keys, vals, d_off, d_ids, nblocks, n, xf, stream, small);
+    if (vb == 8) return launch_block_sort<uint32_t, 8>(keys, vals, d_off, d_ids, nblocks, n, xf, stream, small);
   } else if (kb == 8) {
-    if (vb == 0) return launch_block_sort<uint64_t, 0>(keys, vals, d_off, d_ids, nblocks, n, xf, stream);
-    if (vb == 4) return launch_block_sort<uint64_t, 4>(keys, vals, d_off, d_ids, nblocks, n, xf, stream);
-    if (vb == 8) return launch_block_sort<uint64_t, 8>(keys, vals, d_off, d_ids, nblocks, n, xf, stream);
+    if (vb == 0) return launch_block_sort<uint64_t, 0>(keys, vals, d_off, d_ids, nblocks, n, xf, stream, small);
+    if (vb == 4) return launch_block_sort<uint64_t, 4>(keys, vals, d_off, d_ids, nblocks, n, xf, stream, small);
+    if (vb == 8) return launch_block_sort<uint64_t, 8>(keys, vals, d_off, d_ids, nblocks, n, xf, stream, small);
   }
   return hipErrorInvalidValue;
 }
@@ -151,13 +159,15 @@ static hipError_t dispatch_block_sort(int kb, int vb, void* keys, void* vals, co
 hipError_t block_sort(int kb, int vb, void* keys, void* vals, uint32_t n, const XformParams& xf, hipStream_t stream) {
   if (n > block_sort_max(kb, vb)) return hipErrorInvalidValue;
   if (n <= 1) return hipSuccess;
-  return dispatch_block_sort(kb, vb, keys, vals, nullptr, nullptr, 1, n, xf, stream);
+  return dispatch_block_sort(kb, vb, keys, vals, nullptr, nullptr, 1, n, xf, stream, n <= kBlockSortSmall);
 }
 
 hipError_t block_sort_segments(int kb, int vb, void* keys, void* vals, const uint64_t* d_offsets,
-                               const uint32_t* d_seg_ids, uint32_t nblocks, const XformParams& xf, hipStream_t stream) {
+                               const uint32_t* d_seg_ids, uint32_t nblocks, const XformParams& xf, hipStream_t stream,
+                               uint32_t max_len) {
   if (nblocks == 0) return hipSuccess;
-  return dispatch_block_sort(kb, vb, keys, vals, d_offsets, d_seg_ids, nblocks, 0, xf, stream);
+  const bool small = max_len != 0 && max_len <= kBlockSortSmall;
+  return dispatch_block_sort(kb, vb, keys, vals, d_offsets, d_seg_ids, nblocks, 0, xf, stream, small);
 }
 
 }  // namespace gs

@@ -119,10 +119,13 @@ hipError_t block_sort(int key_bytes, int val_bytes, void* keys, void* vals, uint
 
 // Batched variant: workgroup b sorts segment s = d_seg_ids[b] (or b when
 // d_seg_ids is null), i.e. keys[off[s] .. off[s+1]), which must be no longer
-// than block_sort_max.
+// than block_sort_max.  max_len (0: unknown) bounds the segments' length;
+// up to kBlockSortSmall keys a smaller workgroup with a smaller LDS tile is
+// used, so more of them share a CU.
+constexpr uint32_t kBlockSortSmall = 1024;
 hipError_t block_sort_segments(int key_bytes, int val_bytes, void* keys, void* vals, const uint64_t* d_offsets,
                                const uint32_t* d_seg_ids, uint32_t nblocks, const XformParams& xf,
-                               hipStream_t stream);
+                               hipStream_t stream, uint32_t max_len = 0);
 
 // ---------------------------------------------------------------- merge path
 // One 2-way merge job: out[out_off ..) = merge(a[a_off, a_off+a_len), b[b_off, b_off+b_len)),

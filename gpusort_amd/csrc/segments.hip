@@ -41,6 +41,7 @@ struct SegPlanHost {
   uint32_t nseg = 0;
   uint64_t n_total = 0;
   std::vector<uint32_t> small_ids, multi_ids[2], single_ids[2], huge;
+  uint32_t small_len = 0;  // longest segment of the LDS class
   struct Group {
     uint32_t nslots = 0;
     uint64_t tiles = 0, n_keys = 0, tile_keys = 0;
@@ -65,6 +66,7 @@ void build_seg_plan(int kb, int vb, const uint64_t* off, uint32_t nseg, const Se
     if (len <= 1) continue;
     if (len <= small_max) {
       P->small_ids.push_back(s);
+      P->small_len = std::max(P->small_len, (uint32_t)len);
     } else if (len >= kSegRadixMax) {
       P->huge.push_back(s);
     } else {
@@ -130,7 +132,7 @@ hipError_t launch_seg_plan(const SegPlanHost& P, const unsigned char* dm, void* 
   if (!P.small_ids.empty()) {
     e = block_sort_segments(P.kb, P.vb, d_keys, d_vals, reinterpret_cast<const uint64_t*>(dm + P.off_offsets),
                             reinterpret_cast<const uint32_t*>(dm + P.off_small), (uint32_t)P.small_ids.size(), xf,
-                            stream);
+                            stream, P.small_len);
     if (e != hipSuccess) return e;
   }
   for (int g = 1; g >= 0; --g) {
