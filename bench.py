@@ -731,6 +731,8 @@ def main() -> int:
     ap.add_argument("--cpu-n", type=int, default=1 << 28, help="CPU baseline sample size")
     ap.add_argument("--copy-calib", action="store_true",
                     help="only run plain copies of --n uint32 (counter calibration) and exit")
+    ap.add_argument("--only", choices=["vrank", "batched", "multisplit", "global-index"],
+                    help="run one of the other configurations alone (for profiling) and print its JSON")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -750,6 +752,19 @@ def main() -> int:
         copy_calibration(torch, n, args.steps)
         return 0
     h = g.Handle(0)
+
+    if args.only:
+        log(f"[bench] only: {args.only}")
+        if args.only == "vrank":
+            r = run_c3(h, g, torch, not args.no_cpu)
+        elif args.only == "batched":
+            r = run_batched(h, g, torch, args.arrays, 3)
+        elif args.only == "multisplit":
+            r = run_multisplit(h, g, torch, n)
+        else:
+            r = run_global_index(h, g, torch, n // 2)
+        print(json.dumps({"only": args.only, "result": r}), flush=True)
+        return 0
 
     log(f"[bench] headline: KV u32+u32 uniform n={n} steps={args.steps} warmup={args.warmup}")
     head = run_workload(h, g, torch, "uniform", n, args.steps, args.warmup, payload=True, profile=True)
