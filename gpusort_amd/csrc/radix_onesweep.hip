@@ -693,6 +693,26 @@ struct Geometry<uint64_t, 8> {
   static constexpr int THREADS = 1024, ITEMS = 9;
 };
 
+// Smaller tiles for mid-size single arrays, where the large tile leaves CUs
+// idle (2^22 keys-only 32-bit keys are 137 tiles of 30 K for 256 CUs):
+// keys-only 32-bit keys up to 2^23 keys at 1024 x 16, 32-bit keys with
+// 32-bit payloads up to 2^21 keys at 1024 x 10.  MAX_N 0: no smaller tile.
+template <typename U, int VB>
+struct SmallGeometry {
+  static constexpr int THREADS = Geometry<U, VB>::THREADS, ITEMS = Geometry<U, VB>::ITEMS;
+  static constexpr uint64_t MAX_N = 0;
+};
+template <>
+struct SmallGeometry<uint32_t, 0> {
+  static constexpr int THREADS = 1024, ITEMS = 16;
+  static constexpr uint64_t MAX_N = 1ull << 23;
+};
+template <>
+struct SmallGeometry<uint32_t, 4> {
+  static constexpr int THREADS = 1024, ITEMS = 10;
+  static constexpr uint64_t MAX_N = 1ull << 21;
+};
+
 template <int VB>
 struct ValT {
   using T = uint32_t;
@@ -780,9 +800,18 @@ struct RadixLayout {
   size_t status_word;
 };
 
+// Tile of a single-array sort of n keys (SmallGeometry up to its MAX_N).
+static inline uint32_t sort_tile_keys(int kb, int vb, uint64_t n) {
+  if (kb == 4 && vb == 0 && n <= SmallGeometry<uint32_t, 0>::MAX_N)
+    return SmallGeometry<uint32_t, 0>::THREADS * SmallGeometry<uint32_t, 0>::ITEMS;
+  if (kb == 4 && vb == 4 && n <= SmallGeometry<uint32_t, 4>::MAX_N)
+    return SmallGeometry<uint32_t, 4>::THREADS * SmallGeometry<uint32_t, 4>::ITEMS;
+  return tile_keys_for(kb, vb);
+}
+
 static RadixLayout radix_layout(int kb, int vb, uint64_t n) {
   RadixLayout L{};
-  const uint32_t tile = tile_keys_for(kb, vb);
+  const uint32_t tile = sort_tile_keys(kb, vb, n);
   L.tiles = (n + tile - 1) / tile;
   L.passes = kb;  // 8-bit digits: one pass per key byte
   L.status_word = (n < (1ull << 30)) ? 4 : 8;
@@ -833,15 +862,13 @@ static uint32_t tile_group() {
   return (uint32_t)(cache[dev] - 1);
 }
 
-template <typename U, int VB, typename S>
+template <typename U, int VB, typename S, int THREADS, int ITEMS>
 static hipError_t radix_sort_t(U* keys, void* vals_v, uint64_t n, const XformParams& xp, unsigned char* ws,
                                const RadixLayout& L, hipStream_t stream, LaunchTimer* timer) {
   using V = typename ValT<VB>::T;
   // Positions fit 32 bits whenever the 32-bit status word is used (n < 2^30).
   using Idx = typename std::conditional<sizeof(S) == 4, uint32_t, uint64_t>::type;
   constexpr bool KV = VB != 0;
-  constexpr int THREADS = Geometry<U, VB>::THREADS;
-  constexpr int ITEMS = Geometry<U, VB>::ITEMS;
   const Xform<U> xf = xform_of<U>(xp);
   const bool ident = xf.float_mask == 0 && xf.sign_flip == 0 && xf.desc_mask == 0;
   U* keys_tmp = reinterpret_cast<U*>(ws + L.off_keys_tmp);
@@ -897,8 +924,13 @@ static hipError_t radix_sort_t(U* keys, void* vals_v, uint64_t n, const XformPar
 template <typename U, int VB>
 static hipError_t radix_sort_s(U* keys, void* vals, uint64_t n, const XformParams& xp, unsigned char* ws,
                                const RadixLayout& L, hipStream_t stream, LaunchTimer* timer) {
-  if (L.status_word == 4) return radix_sort_t<U, VB, uint32_t>(keys, vals, n, xp, ws, L, stream, timer);
-  return radix_sort_t<U, VB, uint64_t>(keys, vals, n, xp, ws, L, stream, timer);
+  using G = Geometry<U, VB>;
+  using SG = SmallGeometry<U, VB>;
+  if (n <= SG::MAX_N)  // n < 2^30: 32-bit status words
+    return radix_sort_t<U, VB, uint32_t, SG::THREADS, SG::ITEMS>(keys, vals, n, xp, ws, L, stream, timer);
+  if (L.status_word == 4)
+    return radix_sort_t<U, VB, uint32_t, G::THREADS, G::ITEMS>(keys, vals, n, xp, ws, L, stream, timer);
+  return radix_sort_t<U, VB, uint64_t, G::THREADS, G::ITEMS>(keys, vals, n, xp, ws, L, stream, timer);
 }
 
 hipError_t radix_sort(int kb, int vb, void* keys, void* vals, uint64_t n, const XformParams& xf, void* workspace,

@@ -19,9 +19,11 @@ pytestmark = pytest.mark.gpu
 KEY_DTYPES = [np.uint32, np.int32, np.float32, np.uint64, np.int64, np.float64]
 
 # Sizes around the block-sort cap (8192 / 4096 / 2048) and the onesweep tiles
-# (30720 keys-only u32, 19456 u32 + 32-bit payload, 16384 keys-only u64,
-# 13312 with 12-byte key + payload, 9216 u64 + 64-bit payload; 18432, 10240
-# and 6144 for multisplit), plus multi-tile and ragged lengths.
+# (at these sizes 16384 keys-only u32 and 10240 u32 + 32-bit payload, the
+# mid-size tiles; 16384 keys-only u64, 13312 with 12-byte key + payload, 9216
+# u64 + 64-bit payload; 18432, 10240 and 6144 for multisplit; the large
+# 32-bit tiles are covered by the segmented tests and the threshold test
+# below), plus multi-tile and ragged lengths.
 SIZES = [0, 1, 2, 3, 63, 64, 65, 1000, 2047, 2048, 2049, 4095, 4096, 4097, 6143, 6144, 6145, 8191, 8192, 8193,
          9215, 9216, 9217, 10239, 10240, 10241, 12287, 12288, 12289, 13311, 13312, 13313, 16383, 16384, 16385, 18431, 18432, 18433, 19455, 19456, 19457, 24575, 24576, 24577, 26625, 28671, 28672, 28673, 30719, 30720, 30721, 49153, 61441, 100_003, 1 << 20]
 
@@ -73,6 +75,27 @@ def test_sort_pairs_stable(handle, kdtype, vdtype, descending):
         assert_bits_equal(gv, wv, f"values n={n}")
         perm = stable_order(keys, descending)
         assert np.array_equal(gv, vals[perm])
+
+
+# Single-array sorts take smaller tiles up to 2^23 keys (32-bit keys alone)
+# and 2^21 keys (32-bit keys with 32-bit payloads); these sizes sit on both
+# sides of each switch, with partial last tiles in the larger geometry.
+@pytest.mark.parametrize("n", [(1 << 21), (1 << 21) + 1, (1 << 21) + 19457, (1 << 23), (1 << 23) + 1,
+                               (1 << 23) + 30721])
+def test_tile_geometry_thresholds(handle, n):
+    keys = make_keys(np.uint32, n, seed=n + 11)
+    say(f"sort_keys u32 n={n}")
+    t = dev(keys)
+    handle.sort_keys(t)
+    assert_bits_equal(host(t), oracle.merge_sort(keys), f"keys n={n}")
+    kk = keys[datagen.draws(5, n) % np.uint64(n // 16)]
+    vals = np.arange(n, dtype=np.uint32)
+    say(f"sort_pairs u32+u32 n={n}")
+    tk, tv = dev(kk), dev(vals)
+    handle.sort_pairs(tk, tv)
+    perm = stable_order(kk, False)
+    assert_bits_equal(host(tk), kk[perm], f"pair keys n={n}")
+    assert_bits_equal(host(tv), vals[perm], f"pair values n={n}")
 
 
 @pytest.mark.parametrize("dist", ["uniform", "zipf", "sorted", "equal"])
