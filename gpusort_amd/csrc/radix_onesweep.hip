@@ -696,7 +696,8 @@ struct Geometry<uint64_t, 8> {
 // Smaller tiles for mid-size single arrays, where the large tile leaves CUs
 // idle (2^22 keys-only 32-bit keys are 137 tiles of 30 K for 256 CUs):
 // keys-only 32-bit keys up to 2^23 keys at 1024 x 16, 32-bit keys with
-// 32-bit payloads up to 2^21 keys at 1024 x 10.  MAX_N 0: no smaller tile.
+// 32-bit payloads up to 2^21 keys at 1024 x 10, keys-only 64-bit keys up to
+// 2^21 keys at 1024 x 8.  MAX_N 0: no smaller tile.
 template <typename U, int VB>
 struct SmallGeometry {
   static constexpr int THREADS = Geometry<U, VB>::THREADS, ITEMS = Geometry<U, VB>::ITEMS;
@@ -710,6 +711,11 @@ struct SmallGeometry<uint32_t, 0> {
 template <>
 struct SmallGeometry<uint32_t, 4> {
   static constexpr int THREADS = 1024, ITEMS = 10;
+  static constexpr uint64_t MAX_N = 1ull << 21;
+};
+template <>
+struct SmallGeometry<uint64_t, 0> {
+  static constexpr int THREADS = 1024, ITEMS = 8;
   static constexpr uint64_t MAX_N = 1ull << 21;
 };
 
@@ -806,6 +812,8 @@ static inline uint32_t sort_tile_keys(int kb, int vb, uint64_t n) {
     return SmallGeometry<uint32_t, 0>::THREADS * SmallGeometry<uint32_t, 0>::ITEMS;
   if (kb == 4 && vb == 4 && n <= SmallGeometry<uint32_t, 4>::MAX_N)
     return SmallGeometry<uint32_t, 4>::THREADS * SmallGeometry<uint32_t, 4>::ITEMS;
+  if (kb == 8 && vb == 0 && n <= SmallGeometry<uint64_t, 0>::MAX_N)
+    return SmallGeometry<uint64_t, 0>::THREADS * SmallGeometry<uint64_t, 0>::ITEMS;
   return tile_keys_for(kb, vb);
 }
 

@@ -19,8 +19,8 @@ pytestmark = pytest.mark.gpu
 KEY_DTYPES = [np.uint32, np.int32, np.float32, np.uint64, np.int64, np.float64]
 
 # Sizes around the block-sort cap (8192 / 4096 / 2048) and the onesweep tiles
-# (at these sizes 16384 keys-only u32 and 10240 u32 + 32-bit payload, the
-# mid-size tiles; 16384 keys-only u64, 13312 with 12-byte key + payload, 9216
+# (at these sizes 16384 keys-only u32, 10240 u32 + 32-bit payload and 8192
+# keys-only u64, the mid-size tiles; 13312 with 12-byte key + payload, 9216
 # u64 + 64-bit payload; 18432, 10240 and 6144 for multisplit; the large
 # 32-bit tiles are covered by the segmented tests and the threshold test
 # below), plus multi-tile and ragged lengths.
@@ -83,6 +83,11 @@ def test_sort_pairs_stable(handle, kdtype, vdtype, descending):
 @pytest.mark.parametrize("n", [(1 << 21), (1 << 21) + 1, (1 << 21) + 19457, (1 << 23), (1 << 23) + 1,
                                (1 << 23) + 30721])
 def test_tile_geometry_thresholds(handle, n):
+    k64 = make_keys(np.uint64, min(n, (1 << 21) + 16385), seed=n + 13)  # 64-bit keys switch at 2^21
+    say(f"sort_keys u64 n={k64.size}")
+    t64 = dev(k64)
+    handle.sort_keys(t64)
+    assert_bits_equal(host(t64), oracle.merge_sort(k64), f"u64 keys n={k64.size}")
     keys = make_keys(np.uint32, n, seed=n + 11)
     say(f"sort_keys u32 n={n}")
     t = dev(keys)
