@@ -61,6 +61,7 @@ class VrankReport(ctypes.Structure):
         ("n_bucket_sizes", ctypes.c_uint32),
         ("final_rank_sizes", ctypes.POINTER(ctypes.c_uint64)),
         ("max_rank_error", ctypes.c_double),
+        ("splitter_iterations", ctypes.c_uint32),
     ]
 
 
@@ -107,6 +108,8 @@ class RequestInfo(ctypes.Structure):
         ("done_us", ctypes.c_double),
         ("batch", ctypes.c_uint64),
         ("batch_size", ctypes.c_uint32),
+        ("dev_start_us", ctypes.c_double),
+        ("dev_done_us", ctypes.c_double),
     ]
 
 
@@ -456,6 +459,7 @@ class Handle:
             "bucket_sizes": bsz[: rep.n_bucket_sizes].copy(),
             "final_rank_sizes": fin.copy(),
             "max_rank_error": rep.max_rank_error,
+            "splitter_iterations": rep.splitter_iterations,
         }
 
     def generate(self, out, dist: int, seed: int, index_base: int = 0, param: float = 0.0, cdf=None) -> None:

@@ -42,7 +42,7 @@ class Profiler : public LaunchTimer {
 class Arena {
  public:
   enum Slot { kRadix = 0, kStageKeys, kStageVals, kStagePairs, kMerge, kSmall, kPipeKeys, kPipeVals, kPipeAux,
-              kSegAux, kSegWork, kSlots };
+              kSegAux, kSegWork, kHykWork, kSlots };
   Arena() {
     for (int i = 0; i < kSlots; ++i) {
       ptr_[i] = nullptr;

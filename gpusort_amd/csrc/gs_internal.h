@@ -188,29 +188,72 @@ struct KwayDest {
   uint32_t nparts;
   uint32_t pad;
 };
-// Layout of a k-way pass: kway_plan fills each destination's sample,
-// boundary and partition bases, the destination of every sample block (256
-// samples) and each destination's first sample block.
+// Sizes of a k-way pass: destinations, samples, partition boundaries and
+// partitions.  The device copy (KwayTotals) holds the exact counts; the host
+// sizes launches and scratch by the same fields (exact, or upper bounds when
+// the plan is built on the device).
+struct KwaySizes {
+  uint32_t ndests = 0, nsamples = 0, nbounds = 0, nparts = 0;
+};
+using KwayTotals = KwaySizes;
+// Host plan of a k-way pass: kway_plan fills each destination's sample,
+// boundary and partition bases and the totals.
 struct KwayPlan {
   std::vector<KwayDest> dests;
-  std::vector<uint32_t> block_dest;
-  std::vector<uint32_t> dest_block0;
-  uint32_t nsamples = 0, nbounds = 0, nparts = 0;
+  KwaySizes totals;
 };
 void kway_plan(KwayPlan* plan);
+// Upper bounds of a pass over n_elems elements in ndests destinations.
+KwaySizes kway_upper_bounds(uint64_t n_elems, uint32_t ndests);
+// Output elements per sample and samples per partition of the pass.
+constexpr uint32_t kKwaySampleStride = 128;
+constexpr uint32_t kKwaySamplesPerPart = 16;
 // Device scratch of a pass (partition boundaries and compact samples).
-size_t kway_scratch_bytes(const KwayPlan& plan, int kb, int vb);
-// Device copies of the plan's tables, and the scratch.
+size_t kway_scratch_bytes(const KwaySizes& sizes, int kb, int vb);
+// Device tables of a pass: the destinations (with their bases), the exact
+// totals, and the scratch.
 struct KwayDevice {
   const KwayDest* dests;
-  const uint32_t* block_dest;
-  const uint32_t* dest_block0;
+  const KwayTotals* totals;
   void* scratch;
 };
-// Merges every destination from (sk, sv) into (dk, dv).  lex_vxf: when given
-// (and there is a payload) elements compare as (key, payload) pairs.
-hipError_t kway_merge(int kb, int vb, const void* sk, const void* sv, void* dk, void* dv, const KwayPlan& plan,
+// Merges every destination from (sk, sv) into (dk, dv); `sizes` bounds the
+// launches.  lex_vxf: when given (and there is a payload) elements compare as
+// (key, payload) pairs.
+hipError_t kway_merge(int kb, int vb, const void* sk, const void* sv, void* dk, void* dv, const KwaySizes& sizes,
                       const KwayDevice& dev, const XformParams& xf, const XformParams* lex_vxf, hipStream_t stream);
+
+// ---------------------------------------------------------------- HykSort rounds
+// HykSort's rounds over p virtual ranks with every step on the device
+// (hyk_device.hip).  The runs [off[r], off[r+1]) of kbuf[cur] / vbuf[cur]
+// must be sorted (lexicographically by (key, payload) when lex); on return
+// cur names the buffer holding the result.
+struct HykRun {
+  int kb, vb;
+  bool lex;
+  XformParams kx, vx;
+  char* kbuf[2];
+  char* vbuf[2];
+  int cur;
+  uint32_t p, k_way, s;
+  uint64_t n;
+};
+// What the rounds leave for the report: per round, the bucket bounds
+// (block of p * bnd_stride words, rank r at r * (kway + 2): run start,
+// lower_bound of each splitter, run end, element offsets) and the splitters
+// (block of p * spl_stride raw keys / payloads, group g's at g * kway), the
+// selection iterations, and the final rank offsets.
+struct HykResult {
+  uint32_t rounds = 0;
+  std::vector<uint32_t> iterations, kway, n_p;
+  std::vector<uint64_t> bounds, final_off;
+  std::vector<char> spl_k, spl_v;
+  size_t bnd_stride = 0, spl_stride = 0;
+};
+bool hyk_device_supported(uint32_t p, uint32_t k_way, uint32_t s);
+size_t hyk_device_workspace_bytes(uint32_t p, uint32_t k_way, uint32_t s, uint64_t n, int kb, int vb);
+hipError_t hyk_rounds_device(HykRun* run, const uint64_t* h_off, void* workspace, size_t workspace_bytes,
+                             hipStream_t stream, LaunchTimer* timer, HykResult* res);
 
 // ---------------------------------------------------------------- splitting
 // Bucket boundaries of sorted runs.  Run r is keys[run_off[r] .. run_off[r+1])

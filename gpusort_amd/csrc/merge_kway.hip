@@ -35,8 +35,8 @@ namespace gs {
 
 namespace {
 
-constexpr uint32_t kSampleStride = 128;
-constexpr uint32_t kSamplesPerPart = 16;
+constexpr uint32_t kSampleStride = kKwaySampleStride;
+constexpr uint32_t kSamplesPerPart = kKwaySamplesPerPart;
 constexpr uint32_t kKwayCap = (kSamplesPerPart + kKwayMax) * kSampleStride;
 constexpr int kKwayThreads = 1024;
 constexpr int kKwayItems = kKwayCap / kKwayThreads;
@@ -94,10 +94,10 @@ __device__ inline uint32_t find_dest(uint32_t n, uint32_t g, F field) {
 
 // Boundaries 0 (nothing taken) and nparts (every run whole) of each
 // destination.
-__global__ void kway_bounds_init_kernel(const KwayDest* __restrict__ dests, uint32_t ndests,
+__global__ void kway_bounds_init_kernel(const KwayDest* __restrict__ dests, const KwayTotals* __restrict__ tot,
                                         uint64_t* __restrict__ bounds) {
   const uint32_t d = blockIdx.x * blockDim.x + threadIdx.x;
-  if (d >= ndests) return;
+  if (d >= tot->ndests) return;
   const KwayDest& D = dests[d];
   const uint32_t nparts = dests[d].nparts;
   for (uint32_t i = 0; i < kKwayMax; ++i) {
@@ -110,11 +110,13 @@ __global__ void kway_bounds_init_kernel(const KwayDest* __restrict__ dests, uint
 // its element's ordered key and payload, stored compactly at index g.
 template <typename U, typename V, bool LEX>
 __global__ __launch_bounds__(256) void kway_gather_kernel(const U* __restrict__ keys, const V* __restrict__ vals,
-                                                          const KwayDest* __restrict__ dests, uint32_t ndests,
-                                                          uint32_t nsamples, Xform<U> xf, Xform<V> vxf,
-                                                          U* __restrict__ samp_k, V* __restrict__ samp_v) {
+                                                          const KwayDest* __restrict__ dests,
+                                                          const KwayTotals* __restrict__ tot, Xform<U> xf,
+                                                          Xform<V> vxf, U* __restrict__ samp_k,
+                                                          V* __restrict__ samp_v) {
   const uint32_t g = blockIdx.x * 256 + threadIdx.x;
-  if (g >= nsamples) return;
+  if (g >= tot->nsamples) return;
+  const uint32_t ndests = tot->ndests;
   const uint32_t d = find_dest(ndests, g, [&](uint32_t x) { return dests[x].samp_base; });
   const KwayDest& D = dests[d];
   uint64_t t = g - D.samp_base;
@@ -129,22 +131,22 @@ __global__ __launch_bounds__(256) void kway_gather_kernel(const U* __restrict__ 
 }
 
 // Rank of every sample among its destination's samples in merged order
-// (binary searches over the compact samples of the other runs, in lockstep).
-// Block b serves destination block_dest[b], samples samp_base + (b -
-// first block of that destination) * 256 + threadIdx.x.  Every
-// kSamplesPerPart-th sample is a partition boundary: it records its run and
-// position, and for every other run the number of that run's samples that
-// precede it (refined to an element count by kway_refine_kernel).
+// (binary searches over the compact samples of the other runs, in lockstep),
+// one lane per sample of the pass.  Every kSamplesPerPart-th sample is a
+// partition boundary: it records its run and position, and for every other
+// run the number of that run's samples that precede it (refined to an
+// element count by kway_refine_kernel).
 template <typename U, typename V, bool LEX>
 __global__ __launch_bounds__(256) void kway_sample_kernel(const KwayDest* __restrict__ dests,
-                                                          const uint32_t* __restrict__ block_dest,
-                                                          const uint32_t* __restrict__ dest_block0,
+                                                          const KwayTotals* __restrict__ tot,
                                                           const U* __restrict__ samp_k,
                                                           const V* __restrict__ samp_v,
                                                           uint64_t* __restrict__ bounds,
                                                           uint32_t* __restrict__ bound_run) {
   using E = KElem<U, V, LEX>;
-  const uint32_t d = block_dest[blockIdx.x];
+  const uint32_t gs = blockIdx.x * 256 + threadIdx.x;
+  if (gs >= tot->nsamples) return;
+  const uint32_t d = find_dest(tot->ndests, gs, [&](uint32_t x) { return dests[x].samp_base; });
   const KwayDest& D = dests[d];
   const uint32_t m = D.m;
   uint32_t sfirst[kKwayMax + 1];  // first compact sample of every run (relative)
@@ -155,7 +157,7 @@ __global__ __launch_bounds__(256) void kway_sample_kernel(const KwayDest* __rest
     acc += r < m ? (uint32_t)(D.run_len[r] / kSampleStride) : 0u;
   }
   sfirst[kKwayMax] = acc;
-  const uint32_t g = (blockIdx.x - dest_block0[d]) * 256 + threadIdx.x;  // sample within the destination
+  const uint32_t g = gs - D.samp_base;  // sample within the destination
   if (g >= acc) return;
   uint32_t i = 0;
 #pragma unroll
@@ -209,14 +211,14 @@ __global__ __launch_bounds__(256) void kway_sample_kernel(const KwayDest* __rest
 // binary searches over that window (all runs in lockstep).
 template <typename U, typename V, bool LEX>
 __global__ __launch_bounds__(256) void kway_refine_kernel(const U* __restrict__ keys, const V* __restrict__ vals,
-                                                          const KwayDest* __restrict__ dests, uint32_t ndests,
-                                                          uint32_t nbounds, Xform<U> xf, Xform<V> vxf,
-                                                          uint64_t* __restrict__ bounds,
+                                                          const KwayDest* __restrict__ dests,
+                                                          const KwayTotals* __restrict__ tot, Xform<U> xf,
+                                                          Xform<V> vxf, uint64_t* __restrict__ bounds,
                                                           const uint32_t* __restrict__ bound_run) {
   using E = KElem<U, V, LEX>;
   const uint32_t gb = blockIdx.x * 256 + threadIdx.x;
-  if (gb >= nbounds) return;
-  const uint32_t d = find_dest(ndests, gb, [&](uint32_t x) { return dests[x].bnd_base; });
+  if (gb >= tot->nbounds) return;
+  const uint32_t d = find_dest(tot->ndests, gb, [&](uint32_t x) { return dests[x].bnd_base; });
   const KwayDest& D = dests[d];
   const uint32_t b = gb - D.bnd_base;
   if (b == 0 || b >= D.nparts) return;  // first and last boundaries are fixed
@@ -277,7 +279,7 @@ template <typename U, typename V, bool KV, bool LEX>
 __global__ __launch_bounds__(kKwayThreads) void kway_merge_kernel(const U* __restrict__ sk, const V* __restrict__ sv,
                                                                   U* __restrict__ dk, V* __restrict__ dv,
                                                                   const KwayDest* __restrict__ dests,
-                                                                  uint32_t ndests,
+                                                                  const KwayTotals* __restrict__ tot,
                                                                   const uint64_t* __restrict__ bounds, Xform<U> xf,
                                                                   Xform<V> vxf) {
   using E = KElem<U, V, LEX>;
@@ -291,8 +293,9 @@ __global__ __launch_bounds__(kKwayThreads) void kway_merge_kernel(const U* __res
   __shared__ uint32_t s_m;
 
   const uint32_t tid = threadIdx.x;
+  if (blockIdx.x >= tot->nparts) return;  // grid sized by an upper bound
   if (tid == 0) {
-    const uint32_t d = find_dest(ndests, blockIdx.x, [&](uint32_t x) { return dests[x].part_base; });
+    const uint32_t d = find_dest(tot->ndests, blockIdx.x, [&](uint32_t x) { return dests[x].part_base; });
     const KwayDest& D = dests[d];
     const uint64_t q = (uint64_t)D.bnd_base + (blockIdx.x - D.part_base);
     uint32_t e = 0;
@@ -422,55 +425,54 @@ struct KValT<8> {
 };
 
 // Device scratch layout: boundaries, boundary runs, compact sample keys and
-// payloads.
+// payloads, for the pass's upper bounds.
 struct KwayScratch {
   size_t bounds, bound_run, samp_k, samp_v, total;
 };
 
-KwayScratch kway_scratch(const KwayPlan& P, int kb, int vb) {
+KwayScratch kway_scratch(const KwaySizes& B, int kb, int vb) {
   KwayScratch L{};
   size_t o = 0;
   L.bounds = o;
-  o = align_up(o + (size_t)P.nbounds * kKwayMax * sizeof(uint64_t), 256);
+  o = align_up(o + (size_t)B.nbounds * kKwayMax * sizeof(uint64_t), 256);
   L.bound_run = o;
-  o = align_up(o + (size_t)P.nbounds * sizeof(uint32_t), 256);
+  o = align_up(o + (size_t)B.nbounds * sizeof(uint32_t), 256);
   L.samp_k = o;
-  o = align_up(o + (size_t)P.nsamples * kb, 256);
+  o = align_up(o + (size_t)B.nsamples * kb, 256);
   L.samp_v = o;
-  o = align_up(o + (size_t)P.nsamples * (vb ? vb : 4), 256);
+  o = align_up(o + (size_t)B.nsamples * (vb ? vb : 4), 256);
   L.total = o;
   return L;
 }
 
 template <typename U, int VB, bool LEX>
-hipError_t kway_launch(const void* sk, const void* sv, void* dk, void* dv, const KwayPlan& P, const KwayDevice& dev,
+hipError_t kway_launch(const void* sk, const void* sv, void* dk, void* dv, const KwaySizes& B, const KwayDevice& dev,
                        const XformParams& xp, const XformParams& vp, hipStream_t stream) {
   using V = typename KValT<VB>::T;
   constexpr bool KV = VB != 0;
   const Xform<U> xf{(U)xp.float_mask, (U)xp.sign_flip, (U)xp.desc_mask};
   const Xform<V> vxf{(V)vp.float_mask, (V)vp.sign_flip, (V)vp.desc_mask};
-  const KwayScratch L = kway_scratch(P, sizeof(U), VB);
+  const KwayScratch L = kway_scratch(B, sizeof(U), VB);
   char* ws = reinterpret_cast<char*>(dev.scratch);
   uint64_t* bounds = reinterpret_cast<uint64_t*>(ws + L.bounds);
   uint32_t* bound_run = reinterpret_cast<uint32_t*>(ws + L.bound_run);
   U* samp_k = reinterpret_cast<U*>(ws + L.samp_k);
   V* samp_v = reinterpret_cast<V*>(ws + L.samp_v);
-  const uint32_t ndests = (uint32_t)P.dests.size();
-  hipLaunchKernelGGL(kway_bounds_init_kernel, dim3((ndests + 255) / 256), dim3(256), 0, stream, dev.dests, ndests,
-                     bounds);
-  if (P.nsamples) {
-    hipLaunchKernelGGL((kway_gather_kernel<U, V, LEX>), dim3((P.nsamples + 255) / 256), dim3(256), 0, stream,
-                       reinterpret_cast<const U*>(sk), reinterpret_cast<const V*>(sv), dev.dests, ndests, P.nsamples,
-                       xf, vxf, samp_k, samp_v);
-    hipLaunchKernelGGL((kway_sample_kernel<U, V, LEX>), dim3((uint32_t)P.block_dest.size()), dim3(256), 0, stream,
-                       dev.dests, dev.block_dest, dev.dest_block0, samp_k, samp_v, bounds, bound_run);
-    hipLaunchKernelGGL((kway_refine_kernel<U, V, LEX>), dim3((P.nbounds + 255) / 256), dim3(256), 0, stream,
-                       reinterpret_cast<const U*>(sk), reinterpret_cast<const V*>(sv), dev.dests, ndests, P.nbounds,
-                       xf, vxf, bounds, bound_run);
+  hipLaunchKernelGGL(kway_bounds_init_kernel, dim3((B.ndests + 255) / 256), dim3(256), 0, stream, dev.dests,
+                     dev.totals, bounds);
+  if (B.nsamples) {
+    hipLaunchKernelGGL((kway_gather_kernel<U, V, LEX>), dim3((B.nsamples + 255) / 256), dim3(256), 0, stream,
+                       reinterpret_cast<const U*>(sk), reinterpret_cast<const V*>(sv), dev.dests, dev.totals, xf, vxf,
+                       samp_k, samp_v);
+    hipLaunchKernelGGL((kway_sample_kernel<U, V, LEX>), dim3((B.nsamples + 255) / 256), dim3(256), 0, stream,
+                       dev.dests, dev.totals, samp_k, samp_v, bounds, bound_run);
+    hipLaunchKernelGGL((kway_refine_kernel<U, V, LEX>), dim3((B.nbounds + 255) / 256), dim3(256), 0, stream,
+                       reinterpret_cast<const U*>(sk), reinterpret_cast<const V*>(sv), dev.dests, dev.totals, xf, vxf,
+                       bounds, bound_run);
   }
-  hipLaunchKernelGGL((kway_merge_kernel<U, V, KV, LEX>), dim3(P.nparts), dim3(kKwayThreads), 0, stream,
+  hipLaunchKernelGGL((kway_merge_kernel<U, V, KV, LEX>), dim3(B.nparts), dim3(kKwayThreads), 0, stream,
                      reinterpret_cast<const U*>(sk), reinterpret_cast<const V*>(sv), reinterpret_cast<U*>(dk),
-                     reinterpret_cast<V*>(dv), dev.dests, ndests, bounds, xf, vxf);
+                     reinterpret_cast<V*>(dv), dev.dests, dev.totals, bounds, xf, vxf);
   return hipGetLastError();
 }
 
@@ -478,8 +480,6 @@ hipError_t kway_launch(const void* sk, const void* sv, void* dk, void* dv, const
 
 void kway_plan(KwayPlan* P) {
   uint32_t samp = 0, bnd = 0, part = 0;
-  P->block_dest.clear();
-  P->dest_block0.assign(P->dests.size(), 0);
   for (size_t d = 0; d < P->dests.size(); ++d) {
     KwayDest& D = P->dests[d];
     uint64_t ns = 0;
@@ -492,34 +492,42 @@ void kway_plan(KwayPlan* P) {
     D.bnd_base = bnd;
     D.part_base = part;
     D.nparts = (uint32_t)(ns / kSamplesPerPart) + 1;
-    P->dest_block0[d] = (uint32_t)P->block_dest.size();
-    for (uint64_t b = 0; b < (ns + 255) / 256; ++b) P->block_dest.push_back((uint32_t)d);
     samp += (uint32_t)ns;
     bnd += D.nparts + 1;
     part += D.nparts;
   }
-  P->nsamples = samp;
-  P->nbounds = bnd;
-  P->nparts = part;
+  P->totals.ndests = (uint32_t)P->dests.size();
+  P->totals.nsamples = samp;
+  P->totals.nbounds = bnd;
+  P->totals.nparts = part;
 }
 
-size_t kway_scratch_bytes(const KwayPlan& P, int kb, int vb) { return kway_scratch(P, kb, vb).total; }
+KwaySizes kway_upper_bounds(uint64_t n_elems, uint32_t ndests) {
+  KwaySizes B;
+  B.ndests = ndests;
+  B.nsamples = (uint32_t)(n_elems / kSampleStride);
+  B.nparts = (uint32_t)(n_elems / ((uint64_t)kSampleStride * kSamplesPerPart)) + ndests;
+  B.nbounds = B.nparts + ndests;
+  return B;
+}
 
-hipError_t kway_merge(int kb, int vb, const void* sk, const void* sv, void* dk, void* dv, const KwayPlan& P,
+size_t kway_scratch_bytes(const KwaySizes& B, int kb, int vb) { return kway_scratch(B, kb, vb).total; }
+
+hipError_t kway_merge(int kb, int vb, const void* sk, const void* sv, void* dk, void* dv, const KwaySizes& B,
                       const KwayDevice& dev, const XformParams& xf, const XformParams* lex_vxf, hipStream_t stream) {
-  if (P.dests.empty() || P.nparts == 0) return hipSuccess;
+  if (B.ndests == 0 || B.nparts == 0) return hipSuccess;
   const XformParams vz{0, 0, 0};
   const bool lex = lex_vxf != nullptr && vb != 0;
   const XformParams& vp = lex ? *lex_vxf : vz;
 #define GS_KWAY_CASE(UT, VBV)                                                                  \
-  return lex ? kway_launch<UT, VBV, true>(sk, sv, dk, dv, P, dev, xf, vp, stream)              \
-             : kway_launch<UT, VBV, false>(sk, sv, dk, dv, P, dev, xf, vp, stream)
+  return lex ? kway_launch<UT, VBV, true>(sk, sv, dk, dv, B, dev, xf, vp, stream)              \
+             : kway_launch<UT, VBV, false>(sk, sv, dk, dv, B, dev, xf, vp, stream)
   if (kb == 4) {
-    if (vb == 0) return kway_launch<uint32_t, 0, false>(sk, sv, dk, dv, P, dev, xf, vp, stream);
+    if (vb == 0) return kway_launch<uint32_t, 0, false>(sk, sv, dk, dv, B, dev, xf, vp, stream);
     if (vb == 4) GS_KWAY_CASE(uint32_t, 4);
     if (vb == 8) GS_KWAY_CASE(uint32_t, 8);
   } else if (kb == 8) {
-    if (vb == 0) return kway_launch<uint64_t, 0, false>(sk, sv, dk, dv, P, dev, xf, vp, stream);
+    if (vb == 0) return kway_launch<uint64_t, 0, false>(sk, sv, dk, dv, B, dev, xf, vp, stream);
     if (vb == 4) GS_KWAY_CASE(uint64_t, 4);
     if (vb == 8) GS_KWAY_CASE(uint64_t, 8);
   }

@@ -29,6 +29,8 @@
 //      prefix is found, then publish this tile's inclusive prefix
 //   6. LDS is read back in slot order and written to global memory: lanes
 //      that hold the same digit write consecutive addresses.
+#include <atomic>
+
 #include "gs_device.h"
 #include "gs_internal.h"
 
@@ -849,25 +851,34 @@ static Xform<U> xform_of(const XformParams& p) {
   return Xform<U>{(U)p.float_mask, (U)p.sign_flip, (U)p.desc_mask};
 }
 
-// Tiles per XCD group of the single-array pass.  A tile's look-back waits on
-// every lower tile; with grouped hand-out the lower tiles of its super-group
-// may belong to workgroups dispatched up to 8*G positions later, so 8*G must
-// stay well inside the number of resident workgroups (at least one per CU):
-// G is the largest power of two up to 16 with 16*G <= CUs, 0 (one ticket
-// counter) below 16 CUs.
-static uint32_t tile_group() {
-  static int cache[64];
+// Tiles per XCD group of a pass.  A tile's look-back waits on every lower
+// tile.  With one ticket counter taken in arrival order (G = 0) every lower
+// tile is held by a workgroup that has already started, so the pass finishes
+// whatever else runs on the GPU.  With grouped hand-out the lower tiles of a
+// super-group may belong to workgroups dispatched up to 8*G positions later,
+// so progress needs about 8*G resident workgroups of this kernel: true when it
+// has the GPU to itself (at least one per CU), not when other spinning
+// kernels (another sort on another stream) can hold the CUs.  Grouping is
+// therefore used only on the null stream, which the library's own concurrent
+// users (server slots, handles given a stream) never use: G is the largest
+// power of two up to 16 with 16*G <= CUs, 0 below 16 CUs or on any other
+// stream.
+static uint32_t tile_group(hipStream_t stream) {
+  if (stream != nullptr) return 0;
+  static std::atomic<int> cache[64];
   int dev = 0;
   if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return 0;
-  if (cache[dev] == 0) {
+  int known = cache[dev].load(std::memory_order_relaxed);
+  if (known == 0) {
     int cus = 0;
     if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess) cus = 0;
     int g = 0;
     for (int c = 1; c <= 16; c <<= 1)
       if (16 * c <= cus) g = c;
-    cache[dev] = g + 1;  // 0 = not yet known
+    known = g + 1;  // 0 = not yet known
+    cache[dev].store(known, std::memory_order_relaxed);
   }
-  return (uint32_t)(cache[dev] - 1);
+  return (uint32_t)(known - 1);
 }
 
 template <typename U, int VB, typename S, int THREADS, int ITEMS>
@@ -877,6 +888,9 @@ static hipError_t radix_sort_t(U* keys, void* vals_v, uint64_t n, const XformPar
   // Positions fit 32 bits whenever the 32-bit status word is used (n < 2^30).
   using Idx = typename std::conditional<sizeof(S) == 4, uint32_t, uint64_t>::type;
   constexpr bool KV = VB != 0;
+  // The layout (status words, grid) and this launch geometry must agree on
+  // the tile, or the passes would index status words past their region.
+  if (L.tiles != (n + (uint64_t)THREADS * ITEMS - 1) / ((uint64_t)THREADS * ITEMS)) return hipErrorInvalidValue;
   const Xform<U> xf = xform_of<U>(xp);
   const bool ident = xf.float_mask == 0 && xf.sign_flip == 0 && xf.desc_mask == 0;
   U* keys_tmp = reinterpret_cast<U*>(ws + L.off_keys_tmp);
@@ -912,7 +926,7 @@ static hipError_t radix_sort_t(U* keys, void* vals_v, uint64_t n, const XformPar
     a.status = status + p * status_per_pass;
     a.ctrl = ctrl;
     a.tiles = (uint32_t)L.tiles;
-    a.group = tile_group();
+    a.group = tile_group(stream);
     if (timer) timer->begin("onesweep_pass", 2 * n * (sizeof(U) + (KV ? sizeof(V) : 0)), stream);
     if (ident)
       hipLaunchKernelGGL((onesweep_kernel<U, V, KV, THREADS, ITEMS, S, true, false, Idx>), dim3((uint32_t)L.tiles),
@@ -1128,7 +1142,7 @@ static hipError_t radix_segments_t(U* keys, void* vals_v, const SegTiles& T, con
     a.seg_range = T.seg_range;
     a.seg_base = cnt;
     a.passes = L.passes;
-    a.group = tile_group();
+    a.group = tile_group(stream);
     if (timer) timer->begin("seg_onesweep_pass", 2 * T.n_keys * (sizeof(U) + (KV ? sizeof(V) : 0)), stream);
     if (ident)
       hipLaunchKernelGGL((onesweep_kernel<U, V, KV, THREADS, ITEMS, uint32_t, true, true, Idx>),
@@ -1308,7 +1322,7 @@ static hipError_t multisplit_t(const U* keys, const void* vals, uint64_t n, cons
   a.nspl = nspl;
   a.upper = upper;
   a.depth = depth;
-  a.group = tile_group();
+  a.group = tile_group(stream);
   if (timer) timer->begin("multisplit_pass", 2 * n * (sizeof(U) + (KV ? sizeof(V) : 0)), stream);
   hipLaunchKernelGGL((onesweep_kernel<U, V, KV, THREADS, ITEMS, S, false, false, Idx, true>), dim3((uint32_t)L.tiles),
                      dim3(THREADS), 0, stream, a);

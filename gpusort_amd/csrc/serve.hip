@@ -11,9 +11,12 @@
 //   - coalesces small requests of one (key type, payload type, direction) into
 //     one segmented launch: a gather kernel copies them into a staging buffer,
 //     the segmented sort runs, a scatter kernel copies them back;
-//   - runs every other request as its own sort on its slot's stream, so
-//     requests that arrive while others run overlap on the device;
-//   - marks requests done when their slot's event completes.
+//   - runs every other request as its own sort on its slot's stream (one
+//     non-blocking stream per slot), so requests that arrive while others
+//     run overlap on the device;
+//   - marks requests done when their slot's event completes, and reads the
+//     device clock of each batch's start and completion (events on the
+//     slot's stream, against one reference event taken at creation).
 // Requests are device-resident (sorted in place) or in host memory (copied
 // into the slot's staging buffers, sorted, copied back; pinned host memory
 // lets one slot's copies overlap the others' sorts).  The caller must not
@@ -76,6 +79,7 @@ struct Request {
   uint64_t n = 0;
   int desc = 0;
   double submit_us = 0, start_us = -1, done_us = -1;
+  double dev_start_us = -1, dev_done_us = -1;
   gs_status status = GS_OK;
   uint64_t batch = 0;
   uint32_t batch_size = 0;
@@ -84,7 +88,9 @@ struct Request {
 
 struct Slot {
   gs_handle* h = nullptr;
-  hipEvent_t ev = nullptr;
+  hipStream_t stream = nullptr;  // the slot's own non-blocking stream (also h->stream)
+  hipEvent_t ev = nullptr;       // batch completion
+  hipEvent_t ev_start = nullptr; // batch start
   bool busy = false;
   std::vector<uint64_t> reqs;
   std::vector<unsigned char> meta;  // host sources of in-flight uploads
@@ -108,6 +114,7 @@ struct gs_server {
   bool stop = false;
   gs_server_stats st{};
   std::chrono::steady_clock::time_point t0;
+  hipEvent_t ev0 = nullptr;  // device-clock origin of dev_start_us / dev_done_us
   uint64_t next_batch = 1;
 
   size_t pooled_bytes() const {
@@ -151,6 +158,10 @@ gs_status gs_server::launch(Slot& slot, const std::vector<Request>& batch) {
   gs_handle* h = slot.h;
   const Request& r0 = batch[0];
   DeviceScope ds(h->device);
+  {
+    const hipError_t e = hipEventRecord(slot.ev_start, h->stream);
+    if (e != hipSuccess) return from_hip(e, "record start");
+  }
   if (batch.size() == 1 && !r0.host) {
     gs_status s = sort_device(h, r0.kdt, r0.keys, r0.vdt, r0.vals, r0.n, r0.desc);
     if (s != GS_OK) return s;
@@ -319,9 +330,16 @@ void gs_server::run() {
       lk.lock();
       if (q == hipErrorNotReady) continue;
       const double t = now_us(t0);
+      float ms_start = -1.f, ms_done = -1.f;
+      if (q == hipSuccess) {
+        if (hipEventElapsedTime(&ms_start, ev0, s.ev_start) != hipSuccess) ms_start = -1.f;
+        if (hipEventElapsedTime(&ms_done, ev0, s.ev) != hipSuccess) ms_done = -1.f;
+      }
       for (uint64_t id : s.reqs) {
         reqs[id].done = true;
         reqs[id].done_us = t;
+        reqs[id].dev_start_us = ms_start < 0 ? -1.0 : 1e3 * ms_start;
+        reqs[id].dev_done_us = ms_done < 0 ? -1.0 : 1e3 * ms_done;
         if (q != hipSuccess) reqs[id].status = GS_DEVICE_ERROR;
       }
       st.completed += s.reqs.size();
@@ -366,23 +384,44 @@ gs_status gs_server_create(gs_server** out, const gs_server_config* cfg) {
   if (s->cfg.batch_max_requests == 0) s->cfg.batch_max_requests = 4096;
   s->t0 = std::chrono::steady_clock::now();
   s->st.budget = s->cfg.hbm_budget;
+  auto teardown = [s]() {
+    DeviceScope ds(s->cfg.device);
+    for (Slot& o : s->slots) {
+      gs_destroy(o.h);  // waits for the slot's stream
+      if (o.ev) (void)hipEventDestroy(o.ev);
+      if (o.ev_start) (void)hipEventDestroy(o.ev_start);
+      if (o.stream) (void)hipStreamDestroy(o.stream);
+    }
+    if (s->ev0) (void)hipEventDestroy(s->ev0);
+    delete s;
+  };
   for (uint32_t i = 0; i < s->cfg.streams; ++i) {
     Slot slot;
     gs_status st = gs_create(&slot.h, cfg->device);
     if (st == GS_OK) {
       DeviceScope ds(cfg->device);
-      if (hipEventCreateWithFlags(&slot.ev, hipEventDisableTiming) != hipSuccess) st = GS_DEVICE_ERROR;
-    }
-    if (st != GS_OK) {
-      if (slot.h) gs_destroy(slot.h);
-      for (Slot& o : s->slots) {
-        hipEventDestroy(o.ev);
-        gs_destroy(o.h);
-      }
-      delete s;
-      return st == GS_DEVICE_ERROR ? fail(st, "event creation failed") : st;
+      hipError_t e = hipStreamCreateWithFlags(&slot.stream, hipStreamNonBlocking);
+      if (e == hipSuccess) e = hipEventCreate(&slot.ev);
+      if (e == hipSuccess) e = hipEventCreate(&slot.ev_start);
+      if (e == hipSuccess) slot.h->stream = slot.stream;
+      if (e != hipSuccess) st = from_hip(e, "server slot stream and events");
     }
     s->slots.push_back(slot);
+    if (st != GS_OK) {
+      teardown();
+      return st;
+    }
+  }
+  {
+    DeviceScope ds(cfg->device);
+    hipError_t e = hipEventCreate(&s->ev0);
+    if (e == hipSuccess) e = hipEventRecord(s->ev0, s->slots[0].stream);
+    if (e == hipSuccess) e = hipEventSynchronize(s->ev0);
+    if (e != hipSuccess) {
+      const gs_status st = from_hip(e, "server clock event");
+      teardown();
+      return st;
+    }
   }
   s->th = std::thread([s] { s->run(); });
   *out = s;
@@ -476,6 +515,8 @@ gs_status gs_server_info(gs_server* s, uint64_t id, gs_request_info* out) {
   out->done_us = r.done_us;
   out->batch = r.batch;
   out->batch_size = r.batch_size;
+  out->dev_start_us = r.dev_start_us;
+  out->dev_done_us = r.dev_done_us;
   return GS_OK;
 }
 
@@ -494,10 +535,14 @@ gs_status gs_server_destroy(gs_server* s) {
     s->cv_work.notify_all();
   }
   s->th.join();
+  DeviceScope ds(s->cfg.device);
   for (Slot& o : s->slots) {
-    hipEventDestroy(o.ev);
-    gs_destroy(o.h);
+    gs_destroy(o.h);  // waits for the slot's stream
+    (void)hipEventDestroy(o.ev);
+    (void)hipEventDestroy(o.ev_start);
+    (void)hipStreamDestroy(o.stream);
   }
+  (void)hipEventDestroy(s->ev0);
   delete s;
   return GS_OK;
 }

@@ -27,8 +27,8 @@
 // Splitter rules (the reference draws HykSort / HyperQuickSort samples with
 // rand(), which no re-run can reproduce; these rules are deterministic and
 // the oracle re-runs them exactly):
-//   HykSort         s regular samples run[(j+1)*n/(s+1)] per rank of a group;
-//                   splitter i = sorted_samples[(i+1)*|S|/k - 1], i < k-1
+//   HykSort         the iterative selection of SelectSplitters with regular
+//                   draws, every round on the device (hyk_device.hip)
 //   SampleSort      p-1 samples run[(i+1)*n/p] per rank (samplesort_helper.h:
 //                   402-405); ascending: splitter r = S[(r+1)(p-1)-1],
 //                   descending: S[(r+1)(p-1)] (:411-416)
@@ -274,17 +274,17 @@ gs_status merge_lists(Pipe& P, std::vector<std::vector<Run>> lists, const std::v
     kway_plan(&plan);
     AuxLayout L;
     const size_t o_d = L.add(plan.dests.data(), plan.dests.size() * sizeof(KwayDest));
-    const size_t o_b = L.add(plan.block_dest.data(), plan.block_dest.size() * sizeof(uint32_t));
-    const size_t o_f = L.add(plan.dest_block0.data(), plan.dest_block0.size() * sizeof(uint32_t));
+    const size_t o_t = L.add(&plan.totals, sizeof(KwayTotals));
     char* base = nullptr;
     const int vb = P.kv ? P.vb : 0;
-    gs_status st = upload(P, L, &base, kway_scratch_bytes(plan, P.kb, vb));
+    gs_status st = upload(P, L, &base, kway_scratch_bytes(plan.totals, P.kb, vb));
     if (st != GS_OK) return st;
-    const KwayDevice dev{reinterpret_cast<const KwayDest*>(base + o_d), reinterpret_cast<const uint32_t*>(base + o_b),
-                         reinterpret_cast<const uint32_t*>(base + o_f), base + L.total};
+    const KwayDevice dev{reinterpret_cast<const KwayDest*>(base + o_d), reinterpret_cast<const KwayTotals*>(base + o_t),
+                         base + L.total};
     const int src = P.cur, dst = P.cur ^ 1;
     hipError_t e = kway_merge(P.kb, vb, P.kbuf[src], P.kv ? P.vbuf[src] : nullptr, P.kbuf[dst],
-                              P.kv ? P.vbuf[dst] : nullptr, plan, dev, P.kx, P.kv ? &P.vx : nullptr, P.h->stream);
+                              P.kv ? P.vbuf[dst] : nullptr, plan.totals, dev, P.kx, P.kv ? &P.vx : nullptr,
+                              P.h->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(P.h->stream);
     if (e != hipSuccess) return from_hip(e, "k-way merge");
     P.cur ^= 1;
@@ -360,92 +360,65 @@ void regular_samples(uint64_t o, uint64_t len, uint32_t s, std::vector<uint64_t>
   for (uint32_t j = 0; j < s; ++j) idx->push_back(o + (uint64_t)(j + 1) * len / (s + 1));
 }
 
+// HykSort's rounds, every step on the device (hyk_device.hip); the report
+// is filled from the bounds and splitters the rounds leave behind.
 gs_status hyksort_rounds(Pipe& P, std::vector<uint64_t>& sizes, uint32_t k_way, uint32_t s, Report& rep,
-                         double* max_err, uint32_t* rounds) {
+                         double* max_err, uint32_t* rounds, uint32_t* iterations) {
   const uint32_t p = (uint32_t)sizes.size();
+  if (!hyk_device_supported(p, k_way, s))
+    return fail(GS_UNSUPPORTED, "HykSort with %u ranks, k = %u and %u samples per rank exceeds the device "
+                "selection's limits", p, k_way, s);
+  const size_t ws_bytes = hyk_device_workspace_bytes(p, k_way, s, P.n, P.kb, P.kv ? P.vb : 0);
+  void* ws = nullptr;
+  hipError_t e = P.h->arena.get(Arena::kHykWork, ws_bytes, P.h->stream, &ws);
+  if (e != hipSuccess) return from_hip(e, "HykSort workspace");
+  const std::vector<uint64_t> off = offsets_of(sizes);
+  HykRun run{};
+  run.kb = P.kb;
+  run.vb = P.kv ? P.vb : 0;
+  run.lex = P.kv;
+  run.kx = P.kx;
+  run.vx = P.vx;
+  run.kbuf[0] = P.kbuf[0];
+  run.kbuf[1] = P.kbuf[1];
+  run.vbuf[0] = P.vbuf[0];
+  run.vbuf[1] = P.vbuf[1];
+  run.cur = P.cur;
+  run.p = p;
+  run.k_way = k_way;
+  run.s = s;
+  run.n = P.n;
+  HykResult res;
+  e = hyk_rounds_device(&run, off.data(), ws, ws_bytes, P.h->stream, P.h->timer(), &res);
+  if (e != hipSuccess) return from_hip(e, "HykSort rounds");
+  P.cur = run.cur;
   uint64_t total = 0;
   for (auto z : sizes) total += z;
-  uint32_t n_p = p;
-  while (n_p > 1) {
-    const uint32_t kk = std::min(k_way, n_p);
-    const uint32_t blk = n_p / kk;
-    const uint32_t groups = p / n_p;
-    std::vector<uint64_t> off = offsets_of(sizes);
-    // samples, group by group
-    std::vector<uint64_t> idx, gofs{0};
+  for (uint32_t ri = 0; ri < res.rounds; ++ri) {
+    const uint32_t kway = res.kway[ri], n_p = res.n_p[ri], kk = kway + 1, groups = p / n_p;
+    const uint64_t* bb = res.bounds.data() + (size_t)ri * p * res.bnd_stride;
+    auto B = [&](uint32_t r, uint32_t j) { return bb[(size_t)r * (kway + 2) + j]; };
+    const char* sk = res.spl_k.data() + (size_t)ri * p * res.spl_stride * P.kb;
     for (uint32_t g = 0; g < groups; ++g) {
-      for (uint32_t r = g * n_p; r < (g + 1) * n_p; ++r) regular_samples(off[r], sizes[r], s, &idx);
-      gofs.push_back(idx.size());
-    }
-    std::vector<char> sk, sv;
-    gs_status st = gather_sorted(P, idx, gofs, &sk, &sv);
-    if (st != GS_OK) return st;
-    // splitters: group g's kk-1 splitters; empty groups cut nothing
-    std::vector<char> spl_k((size_t)groups * (kk - 1) * P.kb, 0), spl_v((size_t)groups * (kk - 1) * P.vb, 0);
-    std::vector<bool> group_empty(groups, false);
-    for (uint32_t g = 0; g < groups; ++g) {
-      const uint64_t m = gofs[g + 1] - gofs[g];
-      if (m == 0) {
-        group_empty[g] = true;
-        continue;
-      }
-      std::vector<char> gk;
-      for (uint32_t i = 0; i + 1 < kk; ++i) {
-        int64_t at = (int64_t)((uint64_t)(i + 1) * m / kk) - 1;
-        if (at < 0) at = 0;
-        const uint64_t src = gofs[g] + (uint64_t)at;
-        const size_t dsti = (size_t)g * (kk - 1) + i;
-        memcpy(spl_k.data() + dsti * P.kb, sk.data() + src * P.kb, P.kb);
-        if (P.kv) memcpy(spl_v.data() + dsti * P.vb, sv.data() + src * P.vb, P.vb);
-        gk.insert(gk.end(), sk.data() + src * P.kb, sk.data() + (src + 1) * P.kb);
-      }
-      rep.add_splitters(gk);
-    }
-    std::vector<uint32_t> first(p);
-    for (uint32_t r = 0; r < p; ++r) first[r] = (r / n_p) * (kk - 1);
-    std::vector<uint64_t> bnd;
-    st = run_bounds(P, off, spl_k, spl_v, first, kk - 1, false, &bnd);
-    if (st != GS_OK) return st;
-    auto B = [&](uint32_t r, uint32_t i) { return bnd[(size_t)r * (kk + 1) + i]; };  // i = 0..kk
-    for (uint32_t r = 0; r < p; ++r) {
-      const uint32_t g = r / n_p;
-      for (uint32_t i = 0; i < kk; ++i) rep.add_bucket(group_empty[g] ? 0 : B(r, i + 1) - B(r, i));
-    }
-    // splitter quality: |global rank - (i+1) * N_group / k| / N
-    for (uint32_t g = 0; g < groups; ++g) {
-      if (group_empty[g]) continue;
       uint64_t gtot = 0;
-      for (uint32_t r = g * n_p; r < (g + 1) * n_p; ++r) gtot += sizes[r];
-      for (uint32_t i = 0; i + 1 < kk; ++i) {
+      for (uint32_t r = g * n_p; r < (g + 1) * n_p; ++r) gtot += B(r, kway + 1) - B(r, 0);
+      if (gtot == 0) continue;  // an empty group cuts nothing
+      rep.add_splitters(std::vector<char>(sk + (size_t)g * kway * P.kb, sk + (size_t)(g + 1) * kway * P.kb));
+      // splitter quality: |global rank - (i+1) * N_group / k| / N
+      for (uint32_t i = 0; i < kway; ++i) {
         uint64_t grank = 0;
         for (uint32_t r = g * n_p; r < (g + 1) * n_p; ++r) grank += B(r, i + 1) - B(r, 0);
-        const double err = std::fabs((double)grank - (double)(i + 1) * (double)gtot / kk) / (double)std::max<uint64_t>(total, 1);
+        const double err =
+            std::fabs((double)grank - (double)(i + 1) * (double)gtot / kk) / (double)std::max<uint64_t>(total, 1);
         *max_err = std::max(*max_err, err);
       }
     }
-    // exchange: bucket i of group-local rank lr goes to group-local rank
-    // blk*i + lr % blk (hyksort_helper.h:558, :584); runs merged in source order
-    std::vector<uint64_t> nsizes(p, 0);
-    std::vector<std::vector<Run>> lists(p);
-    for (uint32_t g = 0; g < groups; ++g) {
-      const uint32_t g0 = g * n_p;
-      for (uint32_t dl = 0; dl < n_p; ++dl) {
-        const uint32_t i = dl / blk;
-        for (uint32_t lr = dl % blk; lr < n_p; lr += blk) {
-          const uint32_t r = g0 + lr;
-          const uint64_t a = group_empty[g] ? off[r] : B(r, i);
-          const uint64_t b = group_empty[g] ? (i == 0 && lr == dl ? off[r + 1] : off[r]) : B(r, i + 1);
-          if (b > a) lists[g0 + dl].push_back(Run{a, b - a});
-          nsizes[g0 + dl] += b - a;
-        }
-      }
-    }
-    st = merge_lists(P, lists, offsets_of(nsizes));
-    if (st != GS_OK) return st;
-    sizes = nsizes;
-    n_p = blk;
-    (*rounds)++;
+    for (uint32_t r = 0; r < p; ++r)
+      for (uint32_t i = 0; i < kk; ++i) rep.add_bucket(B(r, i + 1) - B(r, i));
+    *iterations += res.iterations[ri];
   }
+  for (uint32_t r = 0; r < p; ++r) sizes[r] = res.final_off[r + 1] - res.final_off[r];
+  *rounds = res.rounds;
   return GS_OK;
 }
 
@@ -799,6 +772,7 @@ extern "C" gs_status gs_vrank_sort(gs_handle* h, gs_algorithm algo, gs_dtype kdt
     report->n_splitters = 0;
     report->n_bucket_sizes = 0;
     report->max_rank_error = 0.0;
+    report->splitter_iterations = 0;
   }
   Report rep{report, P.kb};
 
@@ -823,7 +797,7 @@ extern "C" gs_status gs_vrank_sort(gs_handle* h, gs_algorithm algo, gs_dtype kdt
     if (st != GS_OK) return st;
   }
 
-  uint32_t rounds = 0;
+  uint32_t rounds = 0, iterations = 0;
   double max_err = 0.0;
   gs_status st = GS_OK;
   if (p > 1) {
@@ -833,7 +807,7 @@ extern "C" gs_status gs_vrank_sort(gs_handle* h, gs_algorithm algo, gs_dtype kdt
       if (st != GS_OK) return st;
     } else {
       if (algo == GS_HYKSORT)
-        st = hyksort_rounds(P, sizes, k_way, samples_per_rank, rep, &max_err, &rounds);
+        st = hyksort_rounds(P, sizes, k_way, samples_per_rank, rep, &max_err, &rounds, &iterations);
       else if (algo == GS_SAMPLESORT)
         st = samplesort_round(P, sizes, rep, &rounds);
       else
@@ -852,6 +826,7 @@ extern "C" gs_status gs_vrank_sort(gs_handle* h, gs_algorithm algo, gs_dtype kdt
   if (report) {
     report->rounds = rounds;
     report->max_rank_error = max_err;
+    report->splitter_iterations = iterations;
     if (report->final_rank_sizes)
       for (uint32_t r = 0; r < p; ++r) report->final_rank_sizes[r] = sizes[r];
   }

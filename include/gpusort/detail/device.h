@@ -129,6 +129,22 @@ void VrankSort(gs_algorithm algo, MpiComm comm, unsigned k_way, int desc, K* key
   comm.SetResult(sizes, images, static_cast<int>(rep.rounds));
 }
 
+// HykSort's key-value local sort with the GPU flag at one rank: the
+// reference runs CudaUtils::SortByKey (include/helpers/hyksort_helper.h:
+// 247-261 -> src/cuda_utils.cu:477, thrust::sort_by_key), which is stable by
+// key, and no exchange round follows a one-rank communicator.
+template <class K>
+void StablePairsOneRank(MpiComm comm, int desc, K* keys, gs_dtype vdt, void* vals, uint64_t n, Thrower empty_as) {
+  constexpr gs_dtype kdt = DTypeOf<K>();
+  static_assert(kdt != GS_NONE, "key type must be a 4- or 8-byte integer or a float / double");
+  if (n == 0) Check(GS_EMPTY_INPUT, empty_as);
+  {
+    std::lock_guard<std::mutex> lock(CallMutex());
+    Check(gs_sort_pairs_host(Handle(), kdt, keys, vdt, vals, n, desc, nullptr, 0, 0), empty_as);
+  }
+  comm.SetResult(std::vector<uint64_t>(1, n), std::vector<uint64_t>(), 0);
+}
+
 // dst[i] = src[idx[i]] for a trivially copyable element of 1, 2, 4 or 8
 // bytes on the device; other element types are gathered on the host.
 template <class Vector>

@@ -33,12 +33,19 @@ void SortWithGlobalIndexes(gs_algorithm algo, MpiComm comm, Comp comp, unsigned 
                            std::vector<Int>* g_indexes, Thrower t);
 
 // One payload of a device type: sorted along as the pair's second member.
+// stable_one_rank: HykSort's GPU branch at one rank, which is stable by key
+// rather than lexicographic (include/helpers/hyksort_helper.h:247-261).
 template <class Comp, class VectorK, class VectorV>
 void SortPairs(gs_algorithm algo, MpiComm comm, Comp, unsigned k_way, VectorK* keys, VectorV* values, Thrower t,
-               std::true_type) {
+               std::true_type, bool stable_one_rank = false) {
   typedef typename VectorK::value_type K;
   typedef typename VectorV::value_type V;
   if (values->size() != keys->size()) throw MpiException("value vector size differs from the key vector size");
+  if (stable_one_rank && comm.Size() == 1) {
+    StablePairsOneRank<K>(comm, DirectionOf<Comp>(), static_cast<K*>(DataOf(keys)), DTypeOf<V>(), DataOf(values),
+                          keys->size(), t);
+    return;
+  }
   VrankSort<K>(algo, comm, k_way, DirectionOf<Comp>(), static_cast<K*>(DataOf(keys)), DTypeOf<V>(), DataOf(values),
                keys->size(), t);
 }
@@ -54,10 +61,24 @@ void SortPairs(gs_algorithm algo, MpiComm comm, Comp comp, unsigned k_way, Vecto
 }
 
 template <class Comp, class VectorK, class VectorV>
+void DispatchPairs(gs_algorithm algo, MpiComm comm, Comp comp, unsigned k_way, VectorK* keys, VectorV* values,
+                   Thrower t, std::true_type tag, bool stable_one_rank) {
+  SortPairs(algo, comm, comp, k_way, keys, values, t, tag, stable_one_rank);
+}
+template <class Comp, class VectorK, class VectorV>
+void DispatchPairs(gs_algorithm algo, MpiComm comm, Comp comp, unsigned k_way, VectorK* keys, VectorV* values,
+                   Thrower t, std::false_type tag, bool) {
+  SortPairs(algo, comm, comp, k_way, keys, values, t, tag);
+}
+
+template <class Comp, class VectorK, class VectorV>
 void SortPairs(gs_algorithm algo, MpiComm comm, Comp comp, unsigned k_way, VectorK* keys, VectorV* values,
-               Thrower t) {
+               Thrower t, bool stable_one_rank = false) {
   typedef typename VectorV::value_type V;
-  SortPairs(algo, comm, comp, k_way, keys, values, t, std::integral_constant<bool, DTypeOf<V>() != GS_NONE>());
+  // A payload sorted through its global index is ordered the same either way
+  // (equal keys keep index order).
+  DispatchPairs(algo, comm, comp, k_way, keys, values, t, std::integral_constant<bool, DTypeOf<V>() != GS_NONE>(),
+                stable_one_rank);
 }
 
 template <class Comp, class VectorK, class Int>

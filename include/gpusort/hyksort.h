@@ -9,8 +9,9 @@
 //    per-rank sizes;
 //  * splitters come from MpiComm::SamplesPerRank() regular samples per rank
 //    instead of rand() sampling (include/helpers/hyksort_helper.h:382-396);
-//  * is_using_gpu is accepted for compatibility: this library has only the
-//    device path, so the sort runs on the GPU either way;
+//  * this library has only the device path, so the sort runs on the GPU
+//    whatever is_using_gpu says; the flag selects the key-value order at one
+//    rank, as the reference's two local-sort branches differ there (below);
 //  * comparators are std::less / std::greater (the orderings the reference's
 //    GPU path is instantiated for);
 //  * no dynamic exception specifications (removed in C++17); the same
@@ -52,11 +53,15 @@ void HykSort(MpiComm comm, unsigned int k_way, bool is_using_gpu, KeyFunc key_fu
   HykSort<KeyType>(comm, std::less<KeyType>(), k_way, is_using_gpu, key_func, list);
 }
 
+// Key-value order: with one rank and is_using_gpu the reference's local sort
+// is CudaUtils::SortByKey, stable by key (include/helpers/hyksort_helper.h:
+// 247-261); otherwise (the CPU branch, or the exchange rounds of more than
+// one rank, which compare pairs through ConvertComp) elements are ordered as
+// (key, value) pairs.
 template <typename StrictWeakOrdering, typename VectorK, typename VectorV>
 detail::EnableIfOrdering<StrictWeakOrdering> HykSort(MpiComm comm, StrictWeakOrdering comp, unsigned int k_way, bool is_using_gpu, VectorK* keys,
              VectorV* values) {
-  (void)is_using_gpu;
-  detail::SortPairs(GS_HYKSORT, comm, comp, k_way, keys, values, detail::Thrower::kThrust);
+  detail::SortPairs(GS_HYKSORT, comm, comp, k_way, keys, values, detail::Thrower::kThrust, is_using_gpu);
 }
 
 template <typename VectorK, typename VectorV>

@@ -142,6 +142,7 @@ typedef struct gs_vrank_report {
   uint32_t n_bucket_sizes;     /* entries written */
   uint64_t* final_rank_sizes;  /* caller buffer, p entries: keys held by each virtual rank at the end */
   double max_rank_error;       /* max |rank(splitter) - target| / N over all rounds (HykSort quality bound) */
+  uint32_t splitter_iterations; /* HykSort: selection iterations, summed over rounds (largest of a round's groups) */
 } gs_vrank_report;
 
 /* Per-rank pipeline of HykSort / SampleSort / HyperQuickSort over p virtual
@@ -274,7 +275,8 @@ gs_status gs_rebalance_sizes(uint64_t* sizes, uint32_t p);
  * A sort server: continuous batching of device-resident sort requests on one
  * GPU.  No counterpart in the reference, whose callers each run their own
  * sort call (src/cuda_utils.cu:97-243) with per-call device allocation.
- * N stream slots with pooled HBM workspace; a scheduler thread admits
+ * N stream slots (each its own non-blocking stream) with pooled HBM
+ * workspace; a scheduler thread admits
  * requests in arrival order when a slot is free and the batch's workspace
  * fits the HBM budget, coalesces small requests of one (key type, payload
  * type, direction) into one segmented launch, runs larger ones on their own
@@ -302,6 +304,11 @@ typedef struct gs_request_info {
   double submit_us, start_us, done_us; /* server clock: submission, launch, completion seen */
   uint64_t batch;                      /* launch the request ran in */
   uint32_t batch_size;                 /* requests in that launch */
+  /* device clock (microseconds from an event taken at server creation):
+   * when the launch's stream reached its start and its completion; -1 until
+   * done.  Launches on different slots overlap here when their streams ran
+   * side by side. */
+  double dev_start_us, dev_done_us;
 } gs_request_info;
 
 typedef struct gs_server_stats {

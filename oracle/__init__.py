@@ -277,5 +277,6 @@ def vrank(keys: np.ndarray, rank_off, vals: np.ndarray | None = None, algorithm:
         "bucket_sizes": bsz[: n_b.value].copy(),
         "final_rank_sizes": fin,
         "max_rank_error": err.value,
+        "splitter_iterations": int(lib().orc_last_splitter_iterations()),
     }
     return kk, vv, rep

@@ -222,10 +222,14 @@ static void keep_no_empty(std::vector<int64_t>& sizes) {
 }
 
 // ---------------------------------------------------------------- virtual ranks
+// Selection iterations of the last run_vrank on this thread.
+static thread_local int g_last_splitter_iters = 0;
+
 struct Report {
   std::vector<int64_t> bucket_sizes;  // per round, per source rank, per bucket
   std::vector<int64_t> final_sizes;
   int rounds = 0;
+  int splitter_iters = 0;  // HykSort: selection iterations, summed over rounds
   double max_err = 0.0;
 };
 
@@ -260,13 +264,112 @@ static std::vector<T> regular_samples(const std::vector<T>& run, int s) {
   return out;
 }
 
-// HykSort rounds (hyksort_helper.h:453-536) with splitter rule:
-//   samples  = s regular samples run[(j+1)*n/(s+1)] of every rank of the group
-//   splitter i = sorted_samples[(i+1)*|S|/k - 1], i < k-1
-// buckets by lower_bound (hyksort_helper.h:563-568); bucket i of group-local
-// rank r goes to group-local rank blk_size*i + r % blk_size (:558, :584);
-// received runs are merged in source-rank order; groups split by
-// blk_id = r / blk_size (:466, :515).
+// HykSort splitter selection: SelectSplitters / SelectSplittersHelper
+// (hyksort_helper.h:335-450) with the random draws replaced by regular ones.
+// Kept from the reference: tol = 0.01 / (k-1) and beta = (int)(cbrt(1/tol)
+// * 3) (:340-344); targets exp_rank[i] = (i+1) * N / k (:348-349); the loop
+// while max_err > (int64)(N * tol) (:351-356); global ranks as sums of
+// per-rank lower_bounds of every sample, over the samples sorted (:403-423);
+// for each splitter the sample just below its target, lb = upper_bound(
+// glb_rank, exp) - 1 clamped to 0, and ub = upper_bound (:427-448), accepted
+// only when its error is below the previous iteration's max_err, which then
+// becomes the largest accepted error (:431, :446-449); an accepted
+// splitter's window in every rank becomes [lower_bound(lb), lower_bound(ub))
+// (:433-443).
+// Deterministic draws: the first iteration takes s regular samples
+// run[(j+1)*n/(s+1)] of every rank's whole run.  Later iterations take, for
+// each splitter, c = min(beta, tot) samples of its windows laid end to end
+// in rank order (tot elements): positions u_j = (2j+1)*tot / (2c), j < c,
+// each mapped to its rank and offset in that rank's window.  (Regular draws
+// at the same fractions of every rank's window would pick nearly equal
+// values from ranks of alike data; laid end to end the ranks' fractions
+// differ.)  Windows only shrink: where the reference sets start to 0 (no
+// sample at or below the target) or end to the run end (no sample above
+// it), the window keeps that bound.  An iteration that draws no sample ends
+// the loop, and so does the 64th.  Returns the number of iterations run.
+constexpr int kHykMaxIters = 64;
+
+template <typename T, typename Comp>
+static int hyk_select_splitters(const std::vector<std::vector<T>>& R, int g0, int n_p, int kk, int s, Comp comp,
+                                std::vector<T>* out) {
+  const int kway = kk - 1;
+  int64_t N = 0;
+  for (int r = g0; r < g0 + n_p; r++) N += (int64_t)R[r].size();
+  const double tol = 1e-2 / kway;
+  const int beta = (int)(std::pow(1.0 / tol, 1.0 / 3.0) * 3.0);
+  std::vector<int64_t> exp_rank(kway);
+  for (int i = 0; i < kway; i++) exp_rank[i] = ((int64_t)(i + 1) * N) / kk;
+  std::vector<std::vector<int64_t>> start(n_p, std::vector<int64_t>(kway, 0)), end(n_p, std::vector<int64_t>(kway));
+  for (int lr = 0; lr < n_p; lr++)
+    for (int i = 0; i < kway; i++) end[lr][i] = (int64_t)R[g0 + lr].size();
+  out->assign(kway, T());
+  int64_t max_err = N;
+  const int64_t cond = (int64_t)(N * tol);
+  int it = 0;
+  while (max_err > cond && it < kHykMaxIters) {
+    std::vector<T> S;
+    if (it == 0) {
+      for (int lr = 0; lr < n_p; lr++) {
+        auto smp = regular_samples<T, Comp>(R[g0 + lr], s);
+        S.insert(S.end(), smp.begin(), smp.end());
+      }
+    } else {
+      for (int i = 0; i < kway; i++) {
+        int64_t tot = 0;
+        for (int lr = 0; lr < n_p; lr++) tot += end[lr][i] - start[lr][i];
+        if (tot == 0) continue;
+        const int64_t c = std::min<int64_t>(beta, tot);
+        int lr = 0;
+        int64_t cum = 0;  // window elements of the ranks before lr
+        for (int64_t j = 0; j < c; j++) {
+          const int64_t u = (2 * j + 1) * tot / (2 * c);
+          while (u >= cum + end[lr][i] - start[lr][i]) {
+            cum += end[lr][i] - start[lr][i];
+            lr++;
+          }
+          S.push_back(R[g0 + lr][start[lr][i] + u - cum]);
+        }
+      }
+    }
+    ++it;
+    if (S.empty()) break;
+    std::sort(S.begin(), S.end(), comp);
+    const int64_t m = (int64_t)S.size();
+    std::vector<std::vector<int64_t>> loc_rank(n_p, std::vector<int64_t>(m));
+    std::vector<int64_t> glb(m, 0);
+    for (int lr = 0; lr < n_p; lr++) {
+      const auto& run = R[g0 + lr];
+      for (int64_t x = 0; x < m; x++) {
+        loc_rank[lr][x] = std::lower_bound(run.begin(), run.end(), S[x], comp) - run.begin();
+        glb[x] += loc_rank[lr][x];
+      }
+    }
+    int64_t new_max = 0;
+    for (int i = 0; i < kway; i++) {
+      const int64_t ub = std::upper_bound(glb.begin(), glb.end(), exp_rank[i]) - glb.begin();
+      const int64_t lb = ub - 1 < 0 ? 0 : ub - 1;
+      const int64_t err = std::llabs(glb[lb] - exp_rank[i]);
+      if (err < max_err) {
+        for (int lr = 0; lr < n_p; lr++) {
+          if (glb[lb] <= exp_rank[i]) start[lr][i] = loc_rank[lr][lb];
+          if (ub < m) end[lr][i] = loc_rank[lr][ub];
+        }
+        (*out)[i] = S[lb];
+        if (new_max < err) new_max = err;
+      }
+    }
+    max_err = new_max;
+  }
+  return it;
+}
+
+// HykSort rounds (hyksort_helper.h:453-536): splitters by
+// hyk_select_splitters over the group; buckets by lower_bound
+// (hyksort_helper.h:563-568); bucket i of group-local rank r goes to
+// group-local rank blk_size*i + r % blk_size (:558, :584); received runs are
+// merged in source-rank order; groups split by blk_id = r / blk_size (:466,
+// :515).  rep.splitter_iters adds up, over the rounds, the largest
+// selection iteration count of the round's groups.
 template <typename T, typename Comp>
 static void vrank_hyksort(std::vector<std::vector<T>>& R, int k_way, int s, Comp comp, Report& rep,
                           std::vector<T>& spl_out, int64_t total) {
@@ -276,19 +379,12 @@ static void vrank_hyksort(std::vector<std::vector<T>>& R, int k_way, int s, Comp
     const int kk = std::min(k_way, n_p);
     const int blk = n_p / kk;
     std::vector<std::vector<T>> NR(p);
+    int round_iters = 0;
     for (int g0 = 0; g0 < p; g0 += n_p) {
-      std::vector<T> S;
-      for (int r = g0; r < g0 + n_p; r++) {
-        auto smp = regular_samples<T, Comp>(R[r], s);
-        S.insert(S.end(), smp.begin(), smp.end());
-      }
-      std::sort(S.begin(), S.end(), comp);
       std::vector<T> spl;
-      for (int i = 0; i < kk - 1; i++) {
-        int64_t idx = S.empty() ? -1 : (int64_t)(i + 1) * (int64_t)S.size() / kk - 1;
-        if (idx < 0) idx = 0;
-        if (!S.empty()) spl.push_back(S[idx]);
-      }
+      int64_t gsz = 0;
+      for (int r = g0; r < g0 + n_p; r++) gsz += (int64_t)R[r].size();
+      if (gsz > 0) round_iters = std::max(round_iters, hyk_select_splitters(R, g0, n_p, kk, s, comp, &spl));
       int64_t gtot = 0;
       for (int r = g0; r < g0 + n_p; r++) gtot += (int64_t)R[r].size();
       if ((int)spl.size() != kk - 1) {
@@ -327,6 +423,7 @@ static void vrank_hyksort(std::vector<std::vector<T>>& R, int k_way, int s, Comp
     R.swap(NR);
     n_p = blk;
     rep.rounds++;
+    rep.splitter_iters += round_iters;
   }
 }
 
@@ -616,6 +713,7 @@ static int run_vrank(int algo, K* keys, V* vals, const int64_t* rank_off, int p,
   for (int64_t i = 0; i < (int64_t)rep.bucket_sizes.size() && i < bucket_cap; i++) bucket_sizes[i] = rep.bucket_sizes[i];
   *max_err = rep.max_err;
   *rounds = rep.rounds;
+  g_last_splitter_iters = rep.splitter_iters;
   return 0;
 }
 
@@ -630,6 +728,7 @@ using namespace orc;
 extern "C" {
 
 void orc_set_threads(int n) { g_threads = n; }
+int orc_last_splitter_iterations(void) { return g_last_splitter_iters; }
 int orc_get_threads(void) { return max_threads(); }
 
 #define DEF_KEYS(kn, K)                                                                          \

@@ -137,6 +137,47 @@ void CaseHykSortPairsLexicographic() {
   Report("hyksort_pairs_desc_lexicographic_p8_k2", ok);
 }
 
+// One rank, tied keys whose payloads run against the order: with is_using_gpu the
+// reference's HykSort runs CudaUtils::SortByKey (stable by key: equal keys
+// keep their input order, include/helpers/hyksort_helper.h:247-261); without
+// it OmpUtils::MergeSort on (key, value) pairs (lexicographic, :262-270).
+// SampleSort never takes the GPU branch, so it stays lexicographic.
+template <class Comp>
+void CaseOneRankPairOrder(const char* name, Comp comp, bool gpu) {
+  const size_t n = 60000;
+  std::vector<int32_t> k = Keys<int32_t>(25, n, 700);
+  // Payloads run against the direction (decreasing for ascending keys,
+  // increasing for descending), so equal keys come out in different orders
+  // under the two rules.
+  const bool asc = std::is_same<Comp, std::less<int32_t>>::value;
+  std::vector<uint32_t> v(n);
+  for (size_t i = 0; i < n; ++i) v[i] = static_cast<uint32_t>(asc ? n - i : i + 1);
+  std::vector<std::pair<int32_t, uint32_t>> stable(n), lexi(n);
+  for (size_t i = 0; i < n; ++i) stable[i] = lexi[i] = {k[i], v[i]};
+  std::stable_sort(stable.begin(), stable.end(), [&](const std::pair<int32_t, uint32_t>& a,
+                                                     const std::pair<int32_t, uint32_t>& b) { return comp(a.first, b.first); });
+  if (std::is_same<Comp, std::less<int32_t>>::value)
+    std::sort(lexi.begin(), lexi.end(), std::less<std::pair<int32_t, uint32_t>>());
+  else
+    std::sort(lexi.begin(), lexi.end(), std::greater<std::pair<int32_t, uint32_t>>());
+  const std::vector<std::pair<int32_t, uint32_t>>& want = gpu ? stable : lexi;
+  std::vector<int32_t> k2 = k;
+  std::vector<uint32_t> v2 = v;
+  MpiComm comm;  // one rank
+  gpusort::HykSort(comm, comp, 4, gpu, &k, &v);
+  bool ok = stable != lexi;  // the two orders really differ on this input
+  for (size_t i = 0; ok && i < n; ++i) ok = k[i] == want[i].first && v[i] == want[i].second;
+  Report(name, ok);
+  if (!gpu) return;
+  gpusort::SampleSort(MpiComm(), comp, &k2, &v2);  // one rank, CPU branch only in the reference
+  std::vector<std::pair<int32_t, uint32_t>> lex(n);
+  for (size_t i = 0; i < n; ++i) lex[i] = {k2[i], v2[i]};
+  const bool sorted = std::is_same<Comp, std::less<int32_t>>::value
+                          ? std::is_sorted(lex.begin(), lex.end(), std::less<std::pair<int32_t, uint32_t>>())
+                          : std::is_sorted(lex.begin(), lex.end(), std::greater<std::pair<int32_t, uint32_t>>());
+  Report((std::string(name) + "_samplesort_lexicographic").c_str(), sorted);
+}
+
 struct Record {
   double weight;
   int32_t id;
@@ -352,6 +393,10 @@ int main(int argc, char** argv) {
   CaseSortByKey();
   CaseHykSortKeys();
   CaseHykSortPairsLexicographic();
+  CaseOneRankPairOrder("hyksort_one_rank_gpu_stable_by_key_asc", std::less<int32_t>(), true);
+  CaseOneRankPairOrder("hyksort_one_rank_gpu_stable_by_key_desc", std::greater<int32_t>(), true);
+  CaseOneRankPairOrder("hyksort_one_rank_cpu_lexicographic_asc", std::less<int32_t>(), false);
+  CaseOneRankPairOrder("hyksort_one_rank_cpu_lexicographic_desc", std::greater<int32_t>(), false);
   CaseHykSortVariadic();
   CaseHykSortKeyFunc();
   CaseNonPow2FallsBackToSampleSort();

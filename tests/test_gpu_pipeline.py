@@ -83,6 +83,9 @@ CASES = [
     ("hyksort", 32, 4, 8, np.float32, True, False, None),
     ("hyksort", 16, 4, 32, np.uint32, True, False, 5),  # heavy ties: empty ranks rebalanced
     ("hyksort", 6, 8, 32, np.uint32, False, False, None),  # p not a power of 2 -> SampleSort
+    ("hyksort", 32, 16, 8, np.uint64, True, False, None),  # 16 runs per destination: two merge passes
+    ("hyksort", 64, 64, 4, np.int32, False, True, None),   # one round of 64 buckets: two merge passes
+    ("hyksort", 16, 2, 1, np.uint32, True, False, None),   # one sample per rank: refined over iterations
     ("samplesort", 8, 0, 0, np.uint32, True, False, None),
     ("samplesort", 64, 0, 0, np.uint64, True, False, None),
     ("samplesort", 12, 0, 0, np.int64, False, True, None),
@@ -117,6 +120,8 @@ def test_vrank_matches_cpu_rule(handle, case):
     assert np.array_equal(got["bucket_sizes"].astype(np.int64), want["bucket_sizes"]), "bucket sizes"
     assert np.array_equal(got["final_rank_sizes"].astype(np.int64), want["final_rank_sizes"]), "final rank sizes"
     assert got["max_rank_error"] == pytest.approx(want["max_rank_error"], rel=1e-12, abs=1e-15)
+    if algo == "hyksort" and p > 1 and (p & (p - 1)) == 0:
+        assert got["splitter_iterations"] == want["splitter_iterations"]
     # the concatenation is the lexicographic sort of the input
     order = np.lexsort((vals, keys)) if kv else np.argsort(keys, kind="stable")
     if desc:
@@ -153,6 +158,12 @@ def test_vrank_large_ranks(handle, case):
     if kv:
         assert_bits_equal(host(tv), want_v, "payloads")
     assert np.array_equal(got["final_rank_sizes"].astype(np.int64), want["final_rank_sizes"]), "final rank sizes"
+    assert_bits_equal(got["splitters"], want["splitters"], "splitters")
+    assert np.array_equal(got["bucket_sizes"].astype(np.int64), want["bucket_sizes"]), "bucket sizes"
+    if algo == "hyksort":
+        assert got["splitter_iterations"] == want["splitter_iterations"]
+        if distinct is None:
+            assert got["max_rank_error"] <= 0.01 / (min(k, p) - 1)
 
 
 SMALL_CASES = [

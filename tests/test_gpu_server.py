@@ -175,3 +175,31 @@ def test_host_and_device_requests_mixed(handle):
         assert same(k if on_host else k.cpu(), k0.cpu())
         if v is not None:
             assert same(v if on_host else v.cpu(), v0.cpu())
+
+
+def test_slots_overlap_on_the_device(handle):
+    # Two large requests submitted together go to two slots; each slot has
+    # its own stream, so on the device clock the second batch starts before
+    # the first completes (on one shared stream it would start after).
+    n = 1 << 26
+    with g.Server(streams=2, coalesce_max_keys=1 << 10) as srv:
+        a = make_request(handle, ("u32", torch.uint32), n, 81)
+        b = make_request(handle, ("u64", None), n, 82)
+        torch.cuda.synchronize()
+        say(f"slot overlap: two requests of {n} keys")
+        ra = srv.submit(*a)
+        rb = srv.submit(*b)
+        srv.drain()
+        ia, ib = srv.info(ra), srv.info(rb)
+    assert ia["batch"] != ib["batch"]
+    for i in (ia, ib):
+        assert i["status"] == 0 and 0 <= i["dev_start_us"] < i["dev_done_us"], i
+    assert max(ia["dev_start_us"], ib["dev_start_us"]) < min(ia["dev_done_us"], ib["dev_done_us"]), (ia, ib)
+    for kind, req, seed in ((("u32", torch.uint32), a, 81), (("u64", None), b, 82)):
+        k0, v0 = make_request(handle, kind, n, seed)
+        if v0 is None:
+            handle.sort_keys(k0)
+        else:
+            handle.sort_pairs(k0, v0)
+        torch.cuda.synchronize()
+        assert same(req[0], k0) and same(req[1], v0)

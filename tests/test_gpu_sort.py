@@ -21,9 +21,9 @@ KEY_DTYPES = [np.uint32, np.int32, np.float32, np.uint64, np.int64, np.float64]
 # Sizes around the block-sort cap (8192 / 4096 / 2048) and the onesweep tiles
 # (at these sizes 16384 keys-only u32, 10240 u32 + 32-bit payload and 8192
 # keys-only u64, the mid-size tiles; 13312 with 12-byte key + payload, 9216
-# u64 + 64-bit payload; 18432, 10240 and 6144 for multisplit; the large
-# 32-bit tiles are covered by the segmented tests and the threshold test
-# below), plus multi-tile and ragged lengths.
+# u64 + 64-bit payload; 18432, 10240 and 6144 for multisplit), plus
+# multi-tile and ragged lengths.  The large single-array tiles are covered for
+# every key type and direction by test_large_tiles_every_type below.
 SIZES = [0, 1, 2, 3, 63, 64, 65, 1000, 2047, 2048, 2049, 4095, 4096, 4097, 6143, 6144, 6145, 8191, 8192, 8193,
          9215, 9216, 9217, 10239, 10240, 10241, 12287, 12288, 12289, 13311, 13312, 13313, 16383, 16384, 16385, 18431, 18432, 18433, 19455, 19456, 19457, 24575, 24576, 24577, 26625, 28671, 28672, 28673, 30719, 30720, 30721, 49153, 61441, 100_003, 1 << 20]
 
@@ -101,6 +101,46 @@ def test_tile_geometry_thresholds(handle, n):
     perm = stable_order(kk, False)
     assert_bits_equal(host(tk), kk[perm], f"pair keys n={n}")
     assert_bits_equal(host(tv), vals[perm], f"pair values n={n}")
+
+
+# The large single-array tiles for every key type and direction (the
+# transformed-key kernels that signed, float and descending sorts use), one
+# size just above each switch: keys alone (32-bit keys past 2^23, 64-bit past
+# 2^21) and 32-bit keys with 32-bit payloads past 2^21.
+@pytest.mark.parametrize("dtype", KEY_DTYPES, ids=lambda d: np.dtype(d).name)
+@pytest.mark.parametrize("descending", [False, True], ids=["asc", "desc"])
+def test_large_tiles_every_type(handle, dtype, descending):
+    d = np.dtype(dtype)
+    n = (1 << 23) + 30721 if d.itemsize == 4 else (1 << 21) + 16385
+    keys = make_keys(dtype, n, seed=n + 3 * d.itemsize + int(descending))
+    say(f"large tile keys {d.name} n={n} desc={descending}")
+    t = dev(keys)
+    handle.sort_keys(t, descending=descending)
+    assert_bits_equal(host(t), oracle.merge_sort(keys, descending=descending), f"keys {d.name} n={n}")
+    nk = (1 << 21) + 19457
+    kk = make_keys(dtype, nk, seed=nk + d.itemsize)
+    kk = kk[datagen.draws(17, nk) % np.uint64(nk // 16)]
+    vals = np.arange(nk, dtype=np.uint32)
+    say(f"large tile pairs {d.name}+u32 n={nk} desc={descending}")
+    tk, tv = dev(kk), dev(vals)
+    handle.sort_pairs(tk, tv, descending=descending)
+    wk, wv = oracle.stable_sort_pairs(kk, vals, descending=descending)
+    assert_bits_equal(host(tk), wk, f"pair keys {d.name} n={nk}")
+    assert_bits_equal(host(tv), wv, f"pair values {d.name} n={nk}")
+
+
+# Skewed keys alone on the large tile: whole waves of one digit take the
+# uniform-wave ranking branch.
+@pytest.mark.parametrize("dist", ["zipf", "equal", "sorted"])
+@pytest.mark.parametrize("descending", [False, True], ids=["asc", "desc"])
+def test_large_tile_skewed_keys(handle, dist, descending):
+    n = (1 << 23) + 30721
+    keys = datagen.keys(dist, 21, n)
+    say(f"large tile skewed keys {dist} n={n} desc={descending}")
+    t = dev(keys)
+    handle.sort_keys(t, descending=descending)
+    want = np.sort(keys)
+    assert_bits_equal(host(t), want[::-1].copy() if descending else want, f"{dist} keys n={n}")
 
 
 @pytest.mark.parametrize("dist", ["uniform", "zipf", "sorted", "equal"])

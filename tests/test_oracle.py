@@ -351,3 +351,225 @@ def test_small_samplesort_network_invariants(p):
         assert list(zip(ok.tolist(), ov.tolist())) == pairs
         assert (rep["final_rank_sizes"] > 0).all() and rep["final_rank_sizes"].sum() == n
         assert rep["rounds"] > 0 and len(rep["splitters"]) == 0
+
+
+# ------------------------------------------------------------------ splitter-rule pins
+# The oracle's SampleSort splitters are a closed form over the sorted samples.
+# The reference does not sort the samples that way: each rank's p-1 samples
+# arr[(i+1)*n/p] (samplesort_helper.h:402-405) go through the distributed
+# bitonic network (BitonicSort, :192-235, :408), then every rank contributes
+# send_splits[npes-2] (ascending) or send_splits[0] (descending) to an
+# Allgather that drops the last (ascending) or first (descending) entry
+# (:411-416).  The oracle's literal network (its N < 5p^2 path) runs that
+# sort here on the sample lists alone, so the check goes through different
+# code than the closed form and catches a wrong sample position or pick index.
+def _local_runs(k, off, v, descending):
+    runs = []
+    for r in range(off.size - 1):
+        a, b = int(off[r]), int(off[r + 1])
+        if v is None:
+            s = np.sort(k[a:b])
+            runs.append((s[::-1].copy() if descending else s, None))
+        else:
+            order = np.lexsort((v[a:b], k[a:b]))
+            if descending:
+                order = order[::-1]
+            runs.append((k[a:b][order], v[a:b][order]))
+    return runs
+
+
+@pytest.mark.parametrize("p", [2, 3, 5, 8, 12, 13, 64])
+@pytest.mark.parametrize("descending", [False, True], ids=["asc", "desc"])
+@pytest.mark.parametrize("with_vals", [False, True], ids=["keys", "pairs"])
+def test_samplesort_splitters_through_the_reference_network(p, descending, with_vals):
+    rng = np.random.default_rng(1000 + p)
+    sizes = rng.integers(5 * p, 9 * p, p)
+    off = np.concatenate([[0], np.cumsum(sizes)]).astype(np.int64)
+    n = int(off[-1])
+    assert n >= 5 * p * p  # the sampling path, not the small-input network
+    k = rng.integers(0, 3 * p, n).astype(np.uint32)  # many ties
+    v = rng.permutation(n).astype(np.uint32) if with_vals else None
+    _, _, rep = oracle.vrank(k, off, v, algorithm="samplesort", descending=descending)
+    # Samples of every rank's sorted run, p-1 each.
+    sk, sv = [], []
+    for rk, rv in _local_runs(k, off, v, descending):
+        m = rk.size
+        idx = [(i + 1) * m // p for i in range(p - 1)]
+        sk.append(rk[idx])
+        if with_vals:
+            sv.append(rv[idx])
+    soff = np.arange(p + 1, dtype=np.int64) * (p - 1)
+    sk_all = np.concatenate(sk)
+    sv_all = np.concatenate(sv) if with_vals else None
+    assert sk_all.size < 5 * p * p  # the network path
+    nk, _, nrep = oracle.vrank(sk_all, soff, sv_all, algorithm="samplesort", descending=descending)
+    # Equal lists stay equal under the network's merge-splits, so every rank
+    # still holds p-1 samples.
+    assert nrep["final_rank_sizes"].tolist() == [p - 1] * p
+    blocks = nk.reshape(p, p - 1)
+    picked = blocks[:, p - 2] if not descending else blocks[:, 0]
+    want = picked[:-1] if not descending else picked[1:]
+    assert np.array_equal(rep["splitters"], want), (rep["splitters"], want)
+
+
+def test_samplesort_network_pick_catches_index_errors():
+    # The comparison above is sensitive: picking the neighbouring sample or
+    # dropping the other end gives different splitters on this input.
+    p = 8
+    rng = np.random.default_rng(3)
+    sizes = rng.integers(5 * p, 9 * p, p)
+    off = np.concatenate([[0], np.cumsum(sizes)]).astype(np.int64)
+    k = rng.integers(0, 10_000, int(off[-1])).astype(np.uint32)
+    _, _, rep = oracle.vrank(k, off, algorithm="samplesort")
+    s = np.sort(np.concatenate([np.sort(k[off[r]:off[r + 1]])[[(i + 1) * int(sizes[r]) // p for i in range(p - 1)]]
+                                for r in range(p)]))
+    right = s[[(r + 1) * (p - 1) - 1 for r in range(p - 1)]]
+    assert np.array_equal(rep["splitters"], right)
+    assert not np.array_equal(rep["splitters"], s[[(r + 1) * (p - 1) for r in range(p - 1)]])
+    assert not np.array_equal(rep["splitters"], s[[(r + 1) * (p - 1) - 2 for r in range(p - 1)]])
+
+
+def test_hyperquicksort_hand_traced_p2():
+    # p = 2, two regular samples per rank at (j+1)*8/3 = positions 2 and 5.
+    # rank 0 sorted [1 2 4 7 8 10 13 15] -> samples 4, 10
+    # rank 1 sorted [0 3 5 6 9 11 12 14] -> samples 5, 11
+    # gathered in rank order [4 10 5 11]; global lower_bound ranks 4 10 5 11;
+    # target 16/2 = 8: distances 4 2 3 3 -> pivot 10 (hyperquicksort_helper.h:251-256).
+    # lower_bound(10): rank 0 keeps [1 2 4 7 8] (5 low, 3 high), rank 1 sends
+    # its low 5 and keeps [11 12 14].
+    k = np.array([13, 2, 7, 10, 4, 15, 1, 8, 6, 12, 3, 14, 9, 0, 11, 5], dtype=np.int32)
+    gk, _, rep = oracle.vrank(k, [0, 8, 16], algorithm="hyperquicksort", samples_per_rank=2)
+    assert gk.tolist() == list(range(16))
+    assert rep["splitters"].tolist() == [10]
+    assert rep["bucket_sizes"].tolist() == [5, 3, 5, 3]
+    assert rep["final_rank_sizes"].tolist() == [10, 6]
+    assert rep["rounds"] == 1
+
+
+def test_hyperquicksort_hand_traced_p4():
+    # p = 4, two samples per rank at (j+1)*len/3, ragged ranks.
+    # Round 1 (one group, 16 keys, target 8):
+    #   r0 [3 8 11 17 20] samples 8 17; r1 [1 6 14] samples 6 14;
+    #   r2 [0 5 9 12 15 19] samples 9 15; r3 [2 18] samples 2 18.
+    #   rank order [8 17 6 14 9 15 2 18], global ranks 6 12 5 10 7 11 2 13,
+    #   distances 2 4 3 2 1 3 6 5 -> pivot 9.  lower_bound(9): 2 2 2 1.
+    #   Ranks 0,1 (<= (4-1)/2) keep their low sides; partners 0<->2, 1<->3
+    #   (hyperquicksort_helper.h:358-372): r0 [0 3 5 8], r1 [1 2 6],
+    #   r2 [9 11 12 15 17 19 20], r3 [14 18].
+    # Round 2, group {0,1} (7 keys, target 3): samples r0 3 5, r1 2 6; rank
+    #   of 3 is 3 -> pivot 3; lower_bound 1 and 2; r0 [0 1 2], r1 [3 5 6 8].
+    # Round 2, group {2,3} (9 keys, target 4): samples r2 12 17, r3 14 18;
+    #   ranks 2 5 3 6, distances 2 1 1 2 -> first best 17; lower_bound 4 and 1;
+    #   r2 [9 11 12 14 15], r3 [17 18 19 20].
+    k = np.array([20, 3, 17, 8, 11, 1, 14, 6, 9, 19, 0, 12, 5, 15, 2, 18], dtype=np.uint64)
+    gk, _, rep = oracle.vrank(k, [0, 5, 8, 14, 16], algorithm="hyperquicksort", samples_per_rank=2)
+    assert gk.tolist() == [0, 1, 2, 3, 5, 6, 8, 9, 11, 12, 14, 15, 17, 18, 19, 20]
+    assert rep["splitters"].tolist() == [9, 3, 17]
+    assert rep["bucket_sizes"].tolist() == [2, 3, 2, 1, 2, 4, 1, 1, 1, 3, 2, 1, 4, 3, 1, 1]
+    assert rep["final_rank_sizes"].tolist() == [3, 4, 5, 4]
+    assert rep["rounds"] == 2
+
+
+def _exchange_rounds_bucket_sizes(k, off, algorithm, k_way, rep, descending=False):
+    # Per (round, source rank, bucket) counts from np.searchsorted on the runs
+    # each round starts from, with the round's reported splitters and the
+    # reference's exchange structure (hyksort_helper.h:546-633,
+    # samplesort_helper.h:504-565, hyperquicksort_helper.h:348-382).
+    p = off.size - 1
+    runs = [np.sort(k[off[r]:off[r + 1]]) for r in range(p)]
+    if descending:
+        runs = [r[::-1].copy() for r in runs]
+    key = (lambda a: -a.astype(np.float64)) if descending else (lambda a: a.astype(np.float64))
+    spl = rep["splitters"].tolist()
+    out = []
+    if algorithm == "samplesort":
+        cuts = [np.concatenate([[0], np.searchsorted(key(r), key(np.array(spl, dtype=k.dtype)), side="right"),
+                                [r.size]]) for r in runs]
+        for c in cuts:
+            out += np.diff(c).tolist()
+        return out
+    n_p, si = p, 0
+    while n_p > 1:
+        kk = min(k_way, n_p) if algorithm == "hyksort" else 2
+        blk = n_p // kk
+        nxt = [None] * p
+        for g0 in range(0, p, n_p):
+            s = np.array(spl[si:si + kk - 1], dtype=k.dtype)
+            si += kk - 1
+            cuts = [np.concatenate([[0], np.searchsorted(key(runs[r]), key(s), side="left"), [runs[r].size]])
+                    for r in range(g0, g0 + n_p)]
+            for c in cuts:
+                out += np.diff(c).tolist()
+            for dl in range(n_p):
+                if algorithm == "hyksort":
+                    i = dl // blk
+                    parts = [runs[g0 + lr][cuts[lr][i]:cuts[lr][i + 1]] for lr in range(dl % blk, n_p, blk)]
+                else:
+                    low = dl <= (n_p - 1) // 2
+                    partner = dl + (n_p // 2 if low else -(n_p // 2))
+                    parts = [runs[g0 + q][cuts[q][0 if low else 1]:cuts[q][1 if low else 2]] for q in (dl, partner)]
+                m = np.sort(np.concatenate(parts))
+                nxt[g0 + dl] = m[::-1].copy() if descending else m
+        runs = nxt
+        n_p = blk
+    return out
+
+
+@pytest.mark.parametrize("algorithm,k_way", [("hyksort", 4), ("hyksort", 2), ("samplesort", 0),
+                                             ("hyperquicksort", 0)])
+@pytest.mark.parametrize("descending", [False, True], ids=["asc", "desc"])
+def test_bucket_sizes_equal_searchsorted_counts(algorithm, k_way, descending):
+    p = 16
+    rng = np.random.default_rng(29)
+    sizes = rng.integers(200, 900, p)
+    off = np.concatenate([[0], np.cumsum(sizes)]).astype(np.int64)
+    k = rng.integers(0, 5000, int(off[-1])).astype(np.int64)
+    _, _, rep = oracle.vrank(k, off, algorithm=algorithm, k_way=max(k_way, 2), samples_per_rank=8,
+                             descending=descending)
+    want = _exchange_rounds_bucket_sizes(k, off, algorithm, k_way, rep, descending)
+    assert rep["bucket_sizes"].tolist() == want
+
+
+def test_hyksort_selection_hand_traced():
+    # p = 2, k = 2 (one splitter): tol = 0.01, beta = (int)(cbrt(100) * 3) = 13,
+    # N = 40, target exp = 20, loop while max_err > (int)(40 * 0.01) = 0.
+    # rank 0 = 0..15, rank 1 = 16..39 (sorted runs; value = global rank).
+    # Iteration 1: one regular sample per rank, run[n/2]: 8 and 28; global
+    #   ranks 8 and 28 -> lb = 8 (error 12), ub = 28.  Windows become
+    #   [lower_bound(8), lower_bound(28)): rank 0 [8, 16), rank 1 [0, 12).
+    # Iteration 2: tot = 8 + 12 = 20, c = min(13, 20) = 13 draws at
+    #   u_j = (2j+1)*20/26 = 0 2 3 5 6 8 10 11 13 14 16 17 19 of the windows
+    #   laid end to end: rank 0 values 8 10 11 13 14, rank 1 values
+    #   16 18 19 21 22 24 25 27.  20 is not drawn: lb = 19 (error 1), ub = 21;
+    #   windows rank 0 [16, 16), rank 1 [3, 5).
+    # Iteration 3: tot = 2, draws u = 0, 1 -> values 19, 20 -> lb = 20, error 0.
+    # (Draws at (j+1)*tot/(c+1) would have hit 20 in iteration 2.)
+    k = np.arange(40, dtype=np.uint32)
+    gk, _, rep = oracle.vrank(k, [0, 16, 40], algorithm="hyksort", k_way=2, samples_per_rank=1)
+    assert gk.tolist() == list(range(40))
+    assert rep["splitters"].tolist() == [20]
+    assert rep["splitter_iterations"] == 3
+    assert rep["bucket_sizes"].tolist() == [16, 0, 4, 20]
+    assert rep["final_rank_sizes"].tolist() == [20, 20]
+    assert rep["max_rank_error"] == 0.0
+
+
+@pytest.mark.parametrize("seed", [1, 2, 3])
+def test_hyksort_selection_reaches_tolerance(seed):
+    # Uniform keys: the refined splitters land within tol = 0.01 / (k-1) of
+    # their targets in every round (hyksort_helper.h:340-356), where the one
+    # regular-sample pick alone misses by about 2.6 % of N at k = 8.
+    p, n, kw = 64, 1 << 20, 8
+    off = np.arange(p + 1, dtype=np.int64) * (n // p)
+    k = datagen.uniform_u64(seed, n)
+    _, _, rep = oracle.vrank(k, off, np.arange(n, dtype=np.uint32), algorithm="hyksort", k_way=kw,
+                             samples_per_rank=32)
+    assert rep["rounds"] == 2
+    assert rep["max_rank_error"] <= 0.01 / (kw - 1)
+    assert 2 <= rep["splitter_iterations"] <= 10
+    # Errors re-measured from the reported splitters with np.searchsorted.
+    runs = [np.sort(k[off[r]:off[r + 1]]) for r in range(p)]
+    spl = rep["splitters"][:kw - 1]
+    glb = [sum(int(np.searchsorted(r, x, side="left")) for r in runs) for x in spl]
+    for i, g in enumerate(glb):
+        assert abs(g - (i + 1) * n / kw) <= 0.01 / (kw - 1) * n
