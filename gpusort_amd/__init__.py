@@ -19,6 +19,8 @@ LIB_PATH = _HERE / "libgpusort_hip.so"
 # gs_dtype codes (include/gpusort_hip.h)
 I32, U32, F32, I64, U64, F64, NONE = 0, 1, 2, 3, 4, 5, 255
 HYKSORT, SAMPLESORT, HYPERQUICKSORT = 0, 1, 2
+# GS_NOALLOC_SCRATCH_BYTES: workspace bytes of gs_sort_pairs_noalloc
+NOALLOC_SCRATCH_BYTES = 72 << 20
 
 STATUS_NAMES = {
     0: "GS_OK",
@@ -176,6 +178,10 @@ _SIGS = {
     "gs_release_arena": ([ctypes.c_void_p], ctypes.c_int),
     "gs_sort_keys": ([ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p, ctypes.c_uint64, ctypes.c_int], ctypes.c_int),
     "gs_sort_pairs": (
+        [ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p, ctypes.c_uint64, ctypes.c_int],
+        ctypes.c_int,
+    ),
+    "gs_sort_pairs_noalloc": (
         [ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p, ctypes.c_uint64, ctypes.c_int],
         ctypes.c_int,
     ),
@@ -361,6 +367,18 @@ class Handle:
             self.lib.gs_sort_pairs(
                 self.h, torch_dtype_code(keys), _dev_ptr(keys), torch_dtype_code(vals), _dev_ptr(vals), keys.numel(),
                 int(descending),
+            )
+        )
+
+    def sort_pairs_noalloc(self, keys, vals=None, descending: bool = False) -> None:
+        """gs_sort_pairs_noalloc: the sort with the library's fixed workspace
+        (NOALLOC_SCRATCH_BYTES per GPU) instead of the handle's arena."""
+        if vals is not None and keys.numel() != vals.numel():
+            raise GpusortError(1, "keys and values differ in length")
+        _check(
+            self.lib.gs_sort_pairs_noalloc(
+                self.h, torch_dtype_code(keys), _dev_ptr(keys), NONE if vals is None else torch_dtype_code(vals),
+                None if vals is None else _dev_ptr(vals), keys.numel(), int(descending),
             )
         )
 

@@ -197,6 +197,7 @@ gs_status gs_create(gs_handle** out, int device) {
   h->stream = nullptr;
   DeviceScope ds(device);
   hipError_t e = hipFree(nullptr);  // initialise the context on this device
+  if (e == hipSuccess) e = noalloc_setup(device);
   if (e != hipSuccess) {
     delete h;
     return from_hip(e, "device init");

@@ -81,6 +81,8 @@ gs_status from_hip(hipError_t e, const char* what);
 bool dtype_ok(gs_dtype d);
 int dtype_bytes(gs_dtype d);
 int val_bytes_of(gs_dtype d);
+// Resolves the no-allocation entry's fixed workspace on `device` (current).
+hipError_t noalloc_setup(int device);
 gs_status sort_device(gs_handle* h, gs_dtype kdt, void* d_keys, gs_dtype vdt, void* d_vals, uint64_t n,
                       int descending);
 // Segmented sort from host offsets, queued on h->stream without waiting;

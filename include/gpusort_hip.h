@@ -92,6 +92,26 @@ gs_status gs_sort_keys(gs_handle* h, gs_dtype kdt, void* d_keys, uint64_t n, int
 gs_status gs_sort_pairs(gs_handle* h, gs_dtype kdt, void* d_keys, gs_dtype vdt, void* d_vals, uint64_t n,
                         int descending);
 
+/* Bytes of the fixed device array that gs_sort_pairs_noalloc uses as its
+ * workspace (one per GPU, held from the first gs_create on that GPU until the
+ * process ends).  It holds the workspace of a sort of 64 MiB of keys and
+ * payloads: e.g. 2^24 u32 keys alone, 2^23 u32 keys with u32 payloads, 2^22
+ * u64 keys with u64 payloads. */
+#define GS_NOALLOC_SCRATCH_BYTES (72ull << 20)
+
+/* gs_sort_pairs (vdt GS_NONE with d_vals NULL: gs_sort_keys) that allocates
+ * nothing: no hipMalloc, hipFree or allocating library call, whatever the
+ * handle's arena holds.  Its workspace is the fixed array above.  No
+ * counterpart in the reference: it is for callers that cannot allocate
+ * device memory (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc).  Limit: a sort whose workspace exceeds
+ * GS_NOALLOC_SCRATCH_BYTES fails with GS_INVALID_ARG.  Calls share the
+ * GPU's array, so calls that overlap on one GPU (other streams, threads or
+ * handles) race; stream-ordered calls on one stream are safe.  Results equal
+ * gs_sort_pairs / gs_sort_keys on the same input. */
+gs_status gs_sort_pairs_noalloc(gs_handle* h, gs_dtype kdt, void* d_keys, gs_dtype vdt, void* d_vals, uint64_t n,
+                                int descending);
+
 /* Host-memory forms: copy in, sort, copy out, synchronised before return.
  * gs_sort_keys_host replaces CudaUtils::Sort(comp, rank, first, last)
  * (include/cuda/cuda_utils.h:16-19, src/cuda_utils.cu:97-167).
