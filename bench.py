@@ -677,6 +677,11 @@ def cpu_baseline(n_sample: int, reps: int = 3):
         "seconds": round(t, 3),
         "verified": ok,
         "cpu_model": _cpu_model(),
+        **_host_cores(),
+        "all_usable_cores": oracle.threads() >= _host_cores()["usable_cpus"],
+        "note": "cores = OpenMP threads the reference's merge sort ran with (omp_get_max_threads); "
+                "usable_cpus = the CPUs this process may run on (its affinity), host_logical_cpus = all of the "
+                "host's logical CPUs",
     }
 
 
@@ -708,6 +713,17 @@ def copy_calibration(torch, n: int, steps: int) -> None:
 
 
 def _cpu_model() -> str:
+    """The host CPU's full model string (lscpu's "Model name", else
+    /proc/cpuinfo's "model name")."""
+    import subprocess
+
+    try:
+        out = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=10).stdout
+        for line in out.splitlines():
+            if line.startswith("Model name:"):
+                return line.split(":", 1)[1].strip()
+    except (OSError, subprocess.SubprocessError):
+        pass
     try:
         with open("/proc/cpuinfo") as f:
             for line in f:
@@ -716,6 +732,17 @@ def _cpu_model() -> str:
     except OSError:
         pass
     return "unknown"
+
+
+def _host_cores() -> dict:
+    """Cores of the host: all logical CPUs, those this process may run on, and
+    the OpenMP thread count the CPU baseline uses."""
+    try:
+        usable = len(os.sched_getaffinity(0))
+    except (AttributeError, OSError):
+        usable = os.cpu_count() or 0
+    return {"host_logical_cpus": os.cpu_count() or 0, "usable_cpus": usable,
+            "omp_num_threads_env": os.environ.get("OMP_NUM_THREADS")}
 
 
 def main() -> int:
@@ -731,7 +758,7 @@ def main() -> int:
     ap.add_argument("--cpu-n", type=int, default=1 << 28, help="CPU baseline sample size")
     ap.add_argument("--copy-calib", action="store_true",
                     help="only run plain copies of --n uint32 (counter calibration) and exit")
-    ap.add_argument("--only", choices=["vrank", "batched", "multisplit", "global-index"],
+    ap.add_argument("--only", choices=["vrank", "batched", "multisplit", "global-index", "keys", "kv"],
                     help="run one of the other configurations alone (for profiling) and print its JSON")
     args = ap.parse_args()
 
@@ -755,7 +782,12 @@ def main() -> int:
 
     if args.only:
         log(f"[bench] only: {args.only}")
-        if args.only == "vrank":
+        if args.only in ("keys", "kv"):
+            r = run_workload(h, g, torch, "uniform", n, args.steps, args.warmup, payload=args.only == "kv",
+                             profile=True)
+            r = {"gkeys_s": round(r["gkeys_s"], 3), "ms": round(r["ms_per_sort"], 4), "verified": r["verified"],
+                 "roofline": roofline_of(r["profile"]), "profile": r["profile"]}
+        elif args.only == "vrank":
             r = run_c3(h, g, torch, not args.no_cpu)
         elif args.only == "batched":
             r = run_batched(h, g, torch, args.arrays, 3)
@@ -779,6 +811,16 @@ def main() -> int:
         r = run_workload(h, g, torch, "uniform", n, 3, 1, payload=False, profile=True)
         extra["keys_only_u32_uniform"] = {"gkeys_s": round(r["gkeys_s"], 3), "ms": round(r["ms_per_sort"], 3),
                                           "verified": r["verified"], "roofline": roofline_of(r["profile"])}
+        log("[bench] KV u32 uniform on a side stream (tiles handed out in arrival order)")
+        side = torch.cuda.Stream()
+        h.set_stream(side.cuda_stream)
+        with torch.cuda.stream(side):
+            r = run_workload(h, g, torch, "uniform", n, 5, 1, payload=True, profile=True)
+        h.set_stream(0)
+        extra["kv_u32_uniform_side_stream"] = {"gkeys_s": round(r["gkeys_s"], 3), "ms": round(r["ms_per_sort"], 3),
+                                               "verified": r["verified"], "roofline": roofline_of(r["profile"]),
+                                               "note": "same sort on a non-null stream: one arrival-order tile "
+                                                       "ticket instead of XCD-grouped tiles"}
         for dist in ["zipf", "sorted", "equal"]:
             log(f"[bench] KV u32 {dist}")
             r = run_workload(h, g, torch, dist, n, 3, 1, payload=True, profile=False)

@@ -149,7 +149,8 @@ gs_status sort_device(gs_handle* h, gs_dtype kdt, void* d_keys, gs_dtype vdt, vo
   void* ws = nullptr;
   hipError_t e = h->arena.get(Arena::kRadix, ws_bytes, h->stream, &ws);
   if (e != hipSuccess) return from_hip(e, "radix workspace");
-  return from_hip(radix_sort(kb, vb, d_keys, d_vals, n, xf, ws, ws_bytes, h->stream, h->timer()), "radix sort");
+  return from_hip(radix_sort(kb, vb, d_keys, d_vals, n, xf, ws, ws_bytes, h->stream, h->timer(), h->tile_policy),
+                  "radix sort");
 }
 
 }  // namespace gs
@@ -346,7 +347,7 @@ gs_status gs_multisplit(gs_handle* h, gs_dtype kdt, const void* d_keys, gs_dtype
   uint64_t* d_base = reinterpret_cast<uint64_t*>(w + base_off);
   if (e == hipSuccess)
     e = multisplit(kb, vb, d_keys, d_vals, n, w + spl_off, nspl, upper ? 1 : 0, xf, d_out_keys, d_out_vals, d_base,
-                   ws, ws_bytes, h->stream, h->timer());
+                   ws, ws_bytes, h->stream, h->timer(), h->tile_policy);
   uint64_t base[256];
   if (e == hipSuccess) e = hipMemcpyAsync(base, d_base, sizeof(base), hipMemcpyDeviceToHost, h->stream);
   if (e == hipSuccess) e = hipStreamSynchronize(h->stream);

@@ -97,6 +97,9 @@ gs_status sort_segments_async(gs_handle* h, gs_dtype kdt, void* d_keys, gs_dtype
 struct gs_handle {
   int device = 0;
   hipStream_t stream = nullptr;
+  // Tile hand-out of the handle's onesweep passes (gs::TilePolicy); the
+  // server grants grouped hand-out to one slot at a time.
+  int tile_policy = gs::kTilesByStream;
   gs::Arena arena;
   gs::Profiler profiler;
   gs::LaunchTimer* timer() { return profiler.active ? &profiler : nullptr; }

@@ -53,13 +53,21 @@ class LaunchTimer {
   virtual void end(hipStream_t stream) = 0;
 };
 
+// Tile hand-out of the onesweep passes.  Grouped hand-out (consecutive tiles
+// on one XCD) needs about 8*G resident workgroups of the pass for its
+// look-back to progress, which only a pass that has the GPU to itself is sure
+// of; the arrival-order ticket progresses whatever else runs.  kTilesByStream:
+// grouped on the null stream, arrival order on any other stream.
+enum TilePolicy : int { kTilesByStream = -1, kTilesArrival = 0, kTilesGrouped = 1 };
+
 // ---------------------------------------------------------------- radix sort
 // Onesweep LSD radix sort of n keys (key_bytes 4/8) with an optional payload
 // (val_bytes 0/4/8), stable.  keys/vals are sorted in place; the workspace
 // holds the ping-pong buffers, histograms and look-back status words.
 size_t radix_workspace_bytes(int key_bytes, int val_bytes, uint64_t n);
 hipError_t radix_sort(int key_bytes, int val_bytes, void* keys, void* vals, uint64_t n, const XformParams& xf,
-                      void* workspace, size_t workspace_bytes, hipStream_t stream, LaunchTimer* timer = nullptr);
+                      void* workspace, size_t workspace_bytes, hipStream_t stream, LaunchTimer* timer = nullptr,
+                      int tile_policy = kTilesByStream);
 
 // Segmented onesweep: many segments of one array sorted by one launch per
 // digit.  Segment slot s is [seg_range[2s], seg_range[2s+1]) of the array,
@@ -89,7 +97,7 @@ size_t multisplit_workspace_bytes(int key_bytes, int val_bytes, uint64_t n);
 hipError_t multisplit(int key_bytes, int val_bytes, const void* keys, const void* vals, uint64_t n,
                       const void* d_splitters, uint32_t nspl, int upper, const XformParams& xf, void* out_keys,
                       void* out_vals, uint64_t* d_bucket_base, void* workspace, size_t workspace_bytes,
-                      hipStream_t stream, LaunchTimer* timer = nullptr);
+                      hipStream_t stream, LaunchTimer* timer = nullptr, int tile_policy = kTilesByStream);
 
 // Keys per tile of segmented-sort geometry `geo` (0 or 1).
 uint32_t radix_seg_tile_keys(int key_bytes, int val_bytes, int geo);
@@ -108,7 +116,7 @@ size_t radix_segments_workspace_bytes(int key_bytes, int val_bytes, uint64_t n_t
                                       uint32_t nslots);
 hipError_t radix_sort_segments(int key_bytes, int val_bytes, void* keys, void* vals, const SegTiles& tiles,
                                const XformParams& xf, void* workspace, size_t workspace_bytes, hipStream_t stream,
-                               LaunchTimer* timer = nullptr);
+                               LaunchTimer* timer = nullptr, int tile_policy = kTilesByStream);
 
 // ---------------------------------------------------------------- block sort
 // Single-workgroup stable sort of n <= block_sort_max(key_bytes, val_bytes)

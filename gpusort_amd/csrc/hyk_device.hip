@@ -111,6 +111,7 @@ struct HykWs {
   U* spl_k;            // [groups][kway] splitter images
   V* spl_v;
   uint32_t* summary;   // [2]: all groups done, largest iteration count
+  uint64_t* loc;       // [groups][cap][n_p] lower_bound of every sample in every rank, or null
 };
 
 // Block-wide exclusive scan of one value per thread (kHykThreads threads);
@@ -273,6 +274,7 @@ __global__ __launch_bounds__(kHykThreads) void hyk_rank_kernel(const U* __restri
   e.k = W.sk[(uint64_t)g * D.cap + x];
   e.v = LEX ? W.sv[(uint64_t)g * D.cap + x] : V(0);
   const uint64_t lb = run_lower_bound<U, V, LEX>(keys, vals, off[r], off[r + 1], e, xf, vxf);
+  if (W.loc) W.loc[((uint64_t)g * D.cap + x) * D.n_p + lr] = lb;
   if (lb) atomicAdd(&W.glb[(uint64_t)g * D.cap + x], (unsigned long long)lb);
 }
 
@@ -379,6 +381,11 @@ __global__ __launch_bounds__(kHykThreads) void hyk_window_kernel(const U* __rest
   const uint32_t r = g * D.n_p + lr;
   const uint64_t idx = (uint64_t)r * D.kway + i;
   const uint64_t base = (uint64_t)g * D.cap;
+  if (W.loc) {  // the rank kernel kept every sample's bound in every rank
+    if (pk.lb_at_or_below) W.wstart[idx] = W.loc[(base + pk.lb) * D.n_p + lr];
+    if (pk.ub != kNoUb) W.wend[idx] = W.loc[(base + pk.ub) * D.n_p + lr];
+    return;
+  }
   if (pk.lb_at_or_below) {
     HElem<U, V, LEX> e;
     e.k = W.sk[base + pk.lb];
@@ -557,8 +564,9 @@ uint32_t pass_chunks(uint32_t kk, uint32_t t) {
 // Layout of the device rounds' workspace.
 struct HykLayout {
   size_t off[2], grp, wstart, wend, cumw, cnt, sk, sv, glb, pick, spl_k, spl_v, summary, spl_raw_k, spl_raw_v,
-      spl_first, bnd, iters, dests, totals, kscratch, total;
+      spl_first, bnd, iters, dests, totals, kscratch, loc, total;
   uint32_t rounds, max_cap, max_kway, max_chunks;
+  bool keep_loc;  // room for every sample's bound in every rank
 };
 
 struct RoundShape {
@@ -589,13 +597,14 @@ HykLayout hyk_layout(uint32_t p, uint32_t k_way, uint32_t s, uint64_t n, int kb,
   HykLayout L{};
   const std::vector<RoundShape> shapes = round_shapes(p, k_way, s);
   L.rounds = (uint32_t)shapes.size();
-  uint64_t sel_cap = 0, sel_cumw = 0, sel_cnt = 0;
+  uint64_t sel_cap = 0, sel_cumw = 0, sel_cnt = 0, loc_words = 0;
   L.max_chunks = 1;
   for (const RoundShape& R : shapes) {
     L.max_chunks = std::max(L.max_chunks, pass_chunks(R.kk, 0));
     L.max_kway = std::max(L.max_kway, R.kway);
     L.max_cap = std::max(L.max_cap, R.cap);
     sel_cap = std::max<uint64_t>(sel_cap, (uint64_t)R.groups * R.cap);
+    loc_words = std::max<uint64_t>(loc_words, (uint64_t)R.groups * R.cap * R.n_p);
     sel_cumw = std::max<uint64_t>(sel_cumw, (uint64_t)R.groups * R.kway * (R.n_p + 1));
     sel_cnt = std::max<uint64_t>(sel_cnt, (uint64_t)R.groups * (R.kway + 1));
   }
@@ -628,6 +637,11 @@ HykLayout hyk_layout(uint32_t p, uint32_t k_way, uint32_t s, uint64_t n, int kb,
   L.dests = take((size_t)p * L.max_chunks * sizeof(KwayDest));
   L.totals = take(sizeof(KwayTotals));
   L.kscratch = take(kway_scratch_bytes(kway_upper_bounds(n, p * L.max_chunks), kb, vb));
+  // Every sample's lower_bound in every rank of its group (groups * cap *
+  // n_p = p * cap words), kept when it takes at most 256 MiB: the window
+  // updates then read them instead of searching the runs again.
+  L.keep_loc = loc_words * 8 <= (256ull << 20);
+  L.loc = L.keep_loc ? take(loc_words * 8) : 0;
   L.total = o;
   return L;
 }
@@ -666,6 +680,7 @@ hipError_t hyk_rounds_t(HykRun* run, const uint64_t* h_off, char* ws, const HykL
   W.spl_k = reinterpret_cast<U*>(ws + L.spl_k);
   W.spl_v = reinterpret_cast<V*>(ws + L.spl_v);
   W.summary = reinterpret_cast<uint32_t*>(ws + L.summary);
+  W.loc = L.keep_loc ? reinterpret_cast<uint64_t*>(ws + L.loc) : nullptr;
   uint32_t* spl_first = reinterpret_cast<uint32_t*>(ws + L.spl_first);
   KwayDest* dests = reinterpret_cast<KwayDest*>(ws + L.dests);
   KwayTotals* totals = reinterpret_cast<KwayTotals*>(ws + L.totals);
@@ -694,7 +709,9 @@ hipError_t hyk_rounds_t(HykRun* run, const uint64_t* h_off, char* ws, const HykL
         (uint32_t)(((uint64_t)R.groups * R.kway * R.n_p + kHykThreads - 1) / kHykThreads);
     uint32_t summary[2] = {0, 0};
     for (uint32_t launched = 0; launched < kHykMaxIters;) {
-      const uint32_t batch = launched == 0 ? 4u : 4u;
+      // Uniform keys converge in about 3 iterations; later batches are
+      // shorter so that a finished loop wastes few launches.
+      const uint32_t batch = launched == 0 ? 3u : 2u;
       for (uint32_t b = 0; b < batch; ++b) {
         hipLaunchKernelGGL((hyk_sample_kernel<U, V, LEX>), dim3(R.groups), dim3(kHykThreads), 0, stream, kc, vc,
                            off[oc], D, W, xf, vxf);

@@ -38,9 +38,11 @@ namespace {
 constexpr uint32_t kSampleStride = kKwaySampleStride;
 constexpr uint32_t kSamplesPerPart = kKwaySamplesPerPart;
 constexpr uint32_t kKwayCap = (kSamplesPerPart + kKwayMax) * kSampleStride;
-constexpr int kKwayThreads = 1024;
-constexpr int kKwayItems = kKwayCap / kKwayThreads;
-static_assert(kKwayItems * kKwayThreads == (int)kKwayCap, "tile covers the cap");
+// 256 threads x 13 outputs: an odd number of elements between consecutive
+// lanes' positions spreads a wave's LDS accesses over the banks.
+constexpr int kKwayThreads = 256;
+constexpr int kKwayItems = 13;
+static_assert(kKwayItems * kKwayThreads >= (int)kKwayCap, "tile covers the cap");
 
 // Element of the merged order: ordered key and, with LEX, ordered payload.
 template <typename U, typename V, bool LEX>
@@ -208,7 +210,7 @@ __global__ __launch_bounds__(256) void kway_sample_kernel(const KwayDest* __rest
 // Exact split of every interior boundary: its element x (run i, position j)
 // is preceded by `c` samples of run r and not by the next, so the number of
 // run r's elements preceding x lies in [c * stride, (c + 1) * stride - 1];
-// binary searches over that window (all runs in lockstep).
+// one lane per (boundary, other run) binary-searches that window.
 template <typename U, typename V, bool LEX>
 __global__ __launch_bounds__(256) void kway_refine_kernel(const U* __restrict__ keys, const V* __restrict__ vals,
                                                           const KwayDest* __restrict__ dests,
@@ -216,57 +218,33 @@ __global__ __launch_bounds__(256) void kway_refine_kernel(const U* __restrict__ 
                                                           Xform<V> vxf, uint64_t* __restrict__ bounds,
                                                           const uint32_t* __restrict__ bound_run) {
   using E = KElem<U, V, LEX>;
-  const uint32_t gb = blockIdx.x * 256 + threadIdx.x;
+  const uint64_t t = (uint64_t)blockIdx.x * 256 + threadIdx.x;
+  const uint32_t gb = (uint32_t)(t / kKwayMax), r = (uint32_t)(t % kKwayMax);
   if (gb >= tot->nbounds) return;
   const uint32_t d = find_dest(tot->ndests, gb, [&](uint32_t x) { return dests[x].bnd_base; });
   const KwayDest& D = dests[d];
   const uint32_t b = gb - D.bnd_base;
   if (b == 0 || b >= D.nparts) return;  // first and last boundaries are fixed
-  const uint32_t m = D.m;
   const uint32_t i = bound_run[gb];
+  if (r >= D.m || r == i) return;
   uint64_t* out = bounds + (uint64_t)gb * kKwayMax;
-  auto elem = [&](uint32_t r, uint64_t pos) {
-    const uint64_t a = D.run_off[r] + pos;
+  auto elem = [&](uint32_t rr, uint64_t pos) {
+    const uint64_t a = D.run_off[rr] + pos;
     return E{to_ordered(keys[a], xf), LEX ? to_ordered(vals[a], vxf) : V(0)};
   };
   const E x = elem(i, out[i]);
-  uint64_t lo[kKwayMax], hi[kKwayMax];
-#pragma unroll
-  for (uint32_t r = 0; r < kKwayMax; ++r) {
-    if (r < m && r != i) {
-      const uint64_t c = out[r];
-      const uint64_t len = D.run_len[r];
-      const uint64_t top = (c + 1) * kSampleStride - 1;
-      lo[r] = c * kSampleStride;
-      hi[r] = top < len ? top : len;
-    } else {
-      lo[r] = hi[r] = 0;
-    }
+  const uint64_t c = out[r];
+  const uint64_t len = D.run_len[r];
+  const uint64_t top = (c + 1) * kSampleStride - 1;
+  uint64_t lo = c * kSampleStride, hi = top < len ? top : len;
+  while (lo < hi) {
+    const uint64_t mid = (lo + hi) >> 1;
+    if (precedes(elem(r, mid), r, x, i))
+      lo = mid + 1;
+    else
+      hi = mid;
   }
-  for (;;) {
-    bool more = false;
-    E y[kKwayMax];
-    uint64_t mid[kKwayMax];
-#pragma unroll
-    for (uint32_t r = 0; r < kKwayMax; ++r) {
-      mid[r] = (lo[r] + hi[r]) >> 1;
-      if (lo[r] < hi[r]) y[r] = elem(r, mid[r]);
-    }
-#pragma unroll
-    for (uint32_t r = 0; r < kKwayMax; ++r) {
-      if (lo[r] < hi[r]) {
-        if (precedes(y[r], r, x, i))
-          lo[r] = mid[r] + 1;
-        else
-          hi[r] = mid[r];
-        more |= lo[r] < hi[r];
-      }
-    }
-    if (!more) break;
-  }
-#pragma unroll
-  for (uint32_t r = 0; r < kKwayMax; ++r)
-    if (r < m && r != i) out[r] = lo[r];
+  out[r] = lo;
 }
 
 // Slots of piece group [r0, r1) of a partition: pieces r0..r1-1 (clamped
@@ -282,11 +260,12 @@ __global__ __launch_bounds__(kKwayThreads) void kway_merge_kernel(const U* __res
                                                                   const KwayTotals* __restrict__ tot,
                                                                   const uint64_t* __restrict__ bounds, Xform<U> xf,
                                                                   Xform<V> vxf) {
-  using E = KElem<U, V, LEX>;
+  constexpr int VT = kKwayItems;
   // Elements (ordered key, payload: its ordered image with LEX) of the
-  // groups merged so far, ping-pong between two levels.
-  __shared__ U s_k[2][kKwayCap];
-  __shared__ V s_v[2][KV ? kKwayCap : 1];
+  // partition, merged level by level in place: every level's outputs are
+  // gathered in registers before any is written back.
+  __shared__ U s_k[kKwayCap];
+  __shared__ V s_v[KV ? kKwayCap : 1];
   __shared__ uint64_t s_src[kKwayMax];  // source element of each piece's slot 0
   __shared__ uint32_t s_start[kKwayMax + 1];
   __shared__ uint64_t s_out;
@@ -320,98 +299,140 @@ __global__ __launch_bounds__(kKwayThreads) void kway_merge_kernel(const U* __res
 
   // Stage the pieces in list order (coalesced within each piece; every
   // thread's loads are issued before any is used).
-  U lk[kKwayItems];
-  V lv[KV ? kKwayItems : 1];
+  {
+    U lk[VT];
+    V lv[KV ? VT : 1];
 #pragma unroll
-  for (int u = 0; u < kKwayItems; ++u) {
-    const uint32_t e = tid + (uint32_t)u * kKwayThreads;
-    if (e < total) {
-      uint32_t r = 0;
+    for (int u = 0; u < VT; ++u) {
+      const uint32_t e = tid + (uint32_t)u * kKwayThreads;
+      if (e < total) {
+        uint32_t r = 0;
 #pragma unroll
-      for (uint32_t c = 1; c < kKwayMax; ++c) r += e >= s_start[c] ? 1u : 0u;
-      const uint64_t a = s_src[r] + e;
-      lk[u] = sk[a];
-      if constexpr (KV) lv[u] = sv[a];
+        for (uint32_t c = 1; c < kKwayMax; ++c) r += e >= s_start[c] ? 1u : 0u;
+        const uint64_t a = s_src[r] + e;
+        lk[u] = sk[a];
+        if constexpr (KV) lv[u] = sv[a];
+      }
     }
-  }
 #pragma unroll
-  for (int u = 0; u < kKwayItems; ++u) {
-    const uint32_t e = tid + (uint32_t)u * kKwayThreads;
-    if (e < total) {
-      s_k[0][e] = to_ordered(lk[u], xf);
-      if constexpr (KV) s_v[0][e] = LEX ? to_ordered(lv[u], vxf) : lv[u];
+    for (int u = 0; u < VT; ++u) {
+      const uint32_t e = tid + (uint32_t)u * kKwayThreads;
+      if (e < total) {
+        s_k[e] = to_ordered(lk[u], xf);
+        if constexpr (KV) s_v[e] = LEX ? to_ordered(lv[u], vxf) : lv[u];
+      }
     }
   }
   __syncthreads();
 
   // Merge tree over the pieces: at level L group j (pieces [j*2^L,
   // (j+1)*2^L)) is the merge of its two halves, the earlier half winning
-  // ties.  Each thread writes kKwayItems consecutive positions of the level's
-  // output, one merge-path search per group its range touches.
-  int cur = 0;
+  // ties.  Thread t produces positions [t*VT, t*VT + VT) of the level: one
+  // merge-path search for its first position, then a serial merge that
+  // reads one element per output; entering the next group (at its first
+  // position, where the path starts at both halves' beginnings) needs no
+  // search.
+  const uint32_t o0 = tid * VT;
   for (uint32_t width = 1; width < m; width <<= 1) {
-    const U* ck = s_k[cur];
-    const V* cv = s_v[cur];
-    U* nk = s_k[cur ^ 1];
-    V* nv = s_v[cur ^ 1];
-    uint32_t o = tid * kKwayItems;
-    const uint32_t o_end = (o + kKwayItems < total) ? o + kKwayItems : total;
-    while (o < o_end) {
+    U rk[VT];
+    V rv[KV ? VT : 1];
+    if (o0 < total) {
       uint32_t g0 = 0;
-      while (group_start(s_start, g0 + 2 * width, m) <= o) g0 += 2 * width;
+      while (group_start(s_start, g0 + 2 * width, m) <= o0) g0 += 2 * width;
       const uint32_t a0 = group_start(s_start, g0, m);
       const uint32_t b0 = group_start(s_start, g0 + width, m);
-      const uint32_t b1 = group_start(s_start, g0 + 2 * width, m);
+      uint32_t b1 = group_start(s_start, g0 + 2 * width, m);
+      const uint32_t diag = o0 - a0;
       const uint32_t na = b0 - a0, nb = b1 - b0;
-      const uint32_t diag = o - a0;
       uint32_t lo = diag > nb ? diag - nb : 0;
-      uint32_t hi = diag < na ? diag : na;
-      while (lo < hi) {
-        const uint32_t mid = (lo + hi) >> 1;
-        const uint32_t pa = a0 + mid, pb = b0 + diag - 1 - mid;
-        const U xa = ck[pa], xb = ck[pb];
-        bool le = xa < xb;
-        if constexpr (LEX) {
-          if (xa == xb) le = cv[pa] <= cv[pb];  // payloads only for equal keys
-        } else {
-          le = xa <= xb;
-        }
-        if (le)
-          lo = mid + 1;
-        else
-          hi = mid;
-      }
-      uint32_t ia = a0 + lo, ib = b0 + (diag - lo);
-      const uint32_t seg_end = o_end < b1 ? o_end : b1;
-      U ka = ia < b0 ? ck[ia] : U(0), kb = ib < b1 ? ck[ib] : U(0);
-      V va = (KV && ia < b0) ? cv[ia] : V(0), vb = (KV && ib < b1) ? cv[ib] : V(0);
-      for (; o < seg_end; ++o) {
-        const bool take_a = ib >= b1 || (ia < b0 && E{ka, LEX ? va : V(0)}.le(E{kb, LEX ? vb : V(0)}));
-        if (take_a) {
-          nk[o] = ka;
-          if constexpr (KV) nv[o] = va;
-          ++ia;
-          if (ia < b0) {
-            ka = ck[ia];
-            if constexpr (KV) va = cv[ia];
+      {
+        uint32_t hi = diag < na ? diag : na;
+        while (lo < hi) {
+          const uint32_t mid = (lo + hi) >> 1;
+          const uint32_t pa = a0 + mid, pb = b0 + diag - 1 - mid;
+          const U xa = s_k[pa], xb = s_k[pb];
+          bool le = xa < xb;
+          if constexpr (LEX) {
+            if (xa == xb) le = s_v[pa] <= s_v[pb];  // payloads only for equal keys
+          } else {
+            le = xa <= xb;
           }
-        } else {
-          nk[o] = kb;
-          if constexpr (KV) nv[o] = vb;
-          ++ib;
-          if (ib < b1) {
-            kb = ck[ib];
-            if constexpr (KV) vb = cv[ib];
+          if (le)
+            lo = mid + 1;
+          else
+            hi = mid;
+        }
+      }
+      // heads of the two halves: lo elements taken from the first
+      uint32_t ia = a0 + lo, ib = b0 + (diag - lo);
+      uint32_t a1 = b0;  // end of the first half
+      U ka = s_k[ia < a1 ? ia : 0], kb = s_k[ib < b1 ? ib : 0];
+      V va = V(0), vb = V(0);
+      if constexpr (KV) {
+        va = s_v[ia < a1 ? ia : 0];
+        vb = s_v[ib < b1 ? ib : 0];
+      }
+      // One LDS read per output, at the next element of the side just
+      // consumed (a side's head is ignored once the side is exhausted).
+#pragma unroll
+      for (int j = 0; j < VT; ++j) {
+        const uint32_t o = o0 + (uint32_t)j;
+        if (o < total) {
+          if (o >= b1) {
+            // Next group (skipping empty ones): its path starts at the
+            // beginnings of its halves.  Rare: at most a few per thread.
+            while (o >= b1) {
+              g0 += 2 * width;
+              ia = b1;
+              a1 = group_start(s_start, g0 + width, m);
+              ib = a1;
+              b1 = group_start(s_start, g0 + 2 * width, m);
+            }
+            ka = s_k[ia < a1 ? ia : 0];
+            kb = s_k[ib < b1 ? ib : 0];
+            if constexpr (KV) {
+              va = s_v[ia < a1 ? ia : 0];
+              vb = s_v[ib < b1 ? ib : 0];
+            }
+          }
+          bool le;
+          if constexpr (LEX)
+            le = ka < kb || (ka == kb && va <= vb);
+          else
+            le = ka <= kb;
+          const bool take_a = ib >= b1 || (ia < a1 && le);
+          rk[j] = take_a ? ka : kb;
+          if constexpr (KV) rv[j] = take_a ? va : vb;
+          ia += take_a ? 1u : 0u;
+          ib += take_a ? 0u : 1u;
+          const uint32_t nx = take_a ? ia : ib;
+          const uint32_t lim = take_a ? a1 : b1;
+          const uint32_t at = nx < lim ? nx : 0u;
+          const U nk = s_k[at];
+          ka = take_a ? nk : ka;
+          kb = take_a ? kb : nk;
+          if constexpr (KV) {
+            const V nv = s_v[at];
+            va = take_a ? nv : va;
+            vb = take_a ? vb : nv;
           }
         }
       }
     }
     __syncthreads();
-    cur ^= 1;
+#pragma unroll
+    for (int j = 0; j < VT; ++j) {
+      const uint32_t o = o0 + (uint32_t)j;
+      if (o < total) {
+        s_k[o] = rk[j];
+        if constexpr (KV) s_v[o] = rv[j];
+      }
+    }
+    __syncthreads();
   }
   for (uint32_t o = tid; o < total; o += kKwayThreads) {
-    dk[out_base + o] = from_ordered(s_k[cur][o], xf);
-    if constexpr (KV) dv[out_base + o] = LEX ? from_ordered(s_v[cur][o], vxf) : s_v[cur][o];
+    dk[out_base + o] = from_ordered(s_k[o], xf);
+    if constexpr (KV) dv[out_base + o] = LEX ? from_ordered(s_v[o], vxf) : s_v[o];
   }
 }
 
@@ -466,7 +487,8 @@ hipError_t kway_launch(const void* sk, const void* sv, void* dk, void* dv, const
                        samp_k, samp_v);
     hipLaunchKernelGGL((kway_sample_kernel<U, V, LEX>), dim3((B.nsamples + 255) / 256), dim3(256), 0, stream,
                        dev.dests, dev.totals, samp_k, samp_v, bounds, bound_run);
-    hipLaunchKernelGGL((kway_refine_kernel<U, V, LEX>), dim3((B.nbounds + 255) / 256), dim3(256), 0, stream,
+    hipLaunchKernelGGL((kway_refine_kernel<U, V, LEX>), dim3(((uint64_t)B.nbounds * kKwayMax + 255) / 256), dim3(256),
+                       0, stream,
                        reinterpret_cast<const U*>(sk), reinterpret_cast<const V*>(sv), dev.dests, dev.totals, xf, vxf,
                        bounds, bound_run);
   }

@@ -74,6 +74,7 @@ extern "C" gs_status gs_sort_pairs_noalloc(gs_handle* h, gs_dtype kdt, void* d_k
   void* ws = scratch_of(h->device);
   if (!ws) return fail(GS_DEVICE_ERROR, "gs_sort_pairs_noalloc: the scratch array of device %d is not resolved",
                        h->device);
-  return from_hip(radix_sort(kb, vb, d_keys, d_vals, n, xf, ws, (size_t)GS_NOALLOC_SCRATCH_BYTES, h->stream, nullptr),
+  return from_hip(radix_sort(kb, vb, d_keys, d_vals, n, xf, ws, (size_t)GS_NOALLOC_SCRATCH_BYTES, h->stream, nullptr,
+                             h->tile_policy),
                   "radix sort");
 }

@@ -859,12 +859,13 @@ static Xform<U> xform_of(const XformParams& p) {
 // so progress needs about 8*G resident workgroups of this kernel: true when it
 // has the GPU to itself (at least one per CU), not when other spinning
 // kernels (another sort on another stream) can hold the CUs.  Grouping is
-// therefore used only on the null stream, which the library's own concurrent
-// users (server slots, handles given a stream) never use: G is the largest
-// power of two up to 16 with 16*G <= CUs, 0 below 16 CUs or on any other
-// stream.
-static uint32_t tile_group(hipStream_t stream) {
-  if (stream != nullptr) return 0;
+// therefore used only when the caller says the pass has the GPU to itself
+// (kTilesGrouped), or by default on the null stream, which the library's own
+// concurrent users (server slots, handles given a stream) never use: G is the
+// largest power of two up to 16 with 16*G <= CUs, 0 below 16 CUs, on any
+// other stream by default, or with kTilesArrival.
+static uint32_t tile_group(hipStream_t stream, int policy) {
+  if (policy == kTilesArrival || (policy == kTilesByStream && stream != nullptr)) return 0;
   static std::atomic<int> cache[64];
   int dev = 0;
   if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return 0;
@@ -883,7 +884,7 @@ static uint32_t tile_group(hipStream_t stream) {
 
 template <typename U, int VB, typename S, int THREADS, int ITEMS>
 static hipError_t radix_sort_t(U* keys, void* vals_v, uint64_t n, const XformParams& xp, unsigned char* ws,
-                               const RadixLayout& L, hipStream_t stream, LaunchTimer* timer) {
+                               const RadixLayout& L, hipStream_t stream, LaunchTimer* timer, int policy) {
   using V = typename ValT<VB>::T;
   // Positions fit 32 bits whenever the 32-bit status word is used (n < 2^30).
   using Idx = typename std::conditional<sizeof(S) == 4, uint32_t, uint64_t>::type;
@@ -926,7 +927,7 @@ static hipError_t radix_sort_t(U* keys, void* vals_v, uint64_t n, const XformPar
     a.status = status + p * status_per_pass;
     a.ctrl = ctrl;
     a.tiles = (uint32_t)L.tiles;
-    a.group = tile_group(stream);
+    a.group = tile_group(stream, policy);
     if (timer) timer->begin("onesweep_pass", 2 * n * (sizeof(U) + (KV ? sizeof(V) : 0)), stream);
     if (ident)
       hipLaunchKernelGGL((onesweep_kernel<U, V, KV, THREADS, ITEMS, S, true, false, Idx>), dim3((uint32_t)L.tiles),
@@ -945,32 +946,32 @@ static hipError_t radix_sort_t(U* keys, void* vals_v, uint64_t n, const XformPar
 
 template <typename U, int VB>
 static hipError_t radix_sort_s(U* keys, void* vals, uint64_t n, const XformParams& xp, unsigned char* ws,
-                               const RadixLayout& L, hipStream_t stream, LaunchTimer* timer) {
+                               const RadixLayout& L, hipStream_t stream, LaunchTimer* timer, int policy) {
   using G = Geometry<U, VB>;
   using SG = SmallGeometry<U, VB>;
   if (n <= SG::MAX_N)  // n < 2^30: 32-bit status words
-    return radix_sort_t<U, VB, uint32_t, SG::THREADS, SG::ITEMS>(keys, vals, n, xp, ws, L, stream, timer);
+    return radix_sort_t<U, VB, uint32_t, SG::THREADS, SG::ITEMS>(keys, vals, n, xp, ws, L, stream, timer, policy);
   if (L.status_word == 4)
-    return radix_sort_t<U, VB, uint32_t, G::THREADS, G::ITEMS>(keys, vals, n, xp, ws, L, stream, timer);
-  return radix_sort_t<U, VB, uint64_t, G::THREADS, G::ITEMS>(keys, vals, n, xp, ws, L, stream, timer);
+    return radix_sort_t<U, VB, uint32_t, G::THREADS, G::ITEMS>(keys, vals, n, xp, ws, L, stream, timer, policy);
+  return radix_sort_t<U, VB, uint64_t, G::THREADS, G::ITEMS>(keys, vals, n, xp, ws, L, stream, timer, policy);
 }
 
 hipError_t radix_sort(int kb, int vb, void* keys, void* vals, uint64_t n, const XformParams& xf, void* workspace,
-                      size_t workspace_bytes, hipStream_t stream, LaunchTimer* timer) {
+                      size_t workspace_bytes, hipStream_t stream, LaunchTimer* timer, int policy) {
   if (n == 0) return hipSuccess;
   const RadixLayout L = radix_layout(kb, vb, n);
   if (workspace_bytes < L.total) return hipErrorInvalidValue;
   unsigned char* ws = reinterpret_cast<unsigned char*>(workspace);
   if (kb == 4) {
     uint32_t* k = reinterpret_cast<uint32_t*>(keys);
-    if (vb == 0) return radix_sort_s<uint32_t, 0>(k, vals, n, xf, ws, L, stream, timer);
-    if (vb == 4) return radix_sort_s<uint32_t, 4>(k, vals, n, xf, ws, L, stream, timer);
-    if (vb == 8) return radix_sort_s<uint32_t, 8>(k, vals, n, xf, ws, L, stream, timer);
+    if (vb == 0) return radix_sort_s<uint32_t, 0>(k, vals, n, xf, ws, L, stream, timer, policy);
+    if (vb == 4) return radix_sort_s<uint32_t, 4>(k, vals, n, xf, ws, L, stream, timer, policy);
+    if (vb == 8) return radix_sort_s<uint32_t, 8>(k, vals, n, xf, ws, L, stream, timer, policy);
   } else if (kb == 8) {
     uint64_t* k = reinterpret_cast<uint64_t*>(keys);
-    if (vb == 0) return radix_sort_s<uint64_t, 0>(k, vals, n, xf, ws, L, stream, timer);
-    if (vb == 4) return radix_sort_s<uint64_t, 4>(k, vals, n, xf, ws, L, stream, timer);
-    if (vb == 8) return radix_sort_s<uint64_t, 8>(k, vals, n, xf, ws, L, stream, timer);
+    if (vb == 0) return radix_sort_s<uint64_t, 0>(k, vals, n, xf, ws, L, stream, timer, policy);
+    if (vb == 4) return radix_sort_s<uint64_t, 4>(k, vals, n, xf, ws, L, stream, timer, policy);
+    if (vb == 8) return radix_sort_s<uint64_t, 8>(k, vals, n, xf, ws, L, stream, timer, policy);
   }
   return hipErrorInvalidValue;
 }
@@ -1091,7 +1092,8 @@ size_t radix_segments_workspace_bytes(int key_bytes, int val_bytes, uint64_t n_t
 
 template <typename U, int VB, typename Idx, int G>
 static hipError_t radix_segments_t(U* keys, void* vals_v, const SegTiles& T, const XformParams& xp,
-                                   unsigned char* ws, const SegLayout& L, hipStream_t stream, LaunchTimer* timer) {
+                                   unsigned char* ws, const SegLayout& L, hipStream_t stream, LaunchTimer* timer,
+                                   int policy) {
   using V = typename ValT<VB>::T;
   constexpr bool KV = VB != 0;
   constexpr int THREADS = SegGeometry<U, VB, G>::THREADS;
@@ -1142,7 +1144,7 @@ static hipError_t radix_segments_t(U* keys, void* vals_v, const SegTiles& T, con
     a.seg_range = T.seg_range;
     a.seg_base = cnt;
     a.passes = L.passes;
-    a.group = tile_group(stream);
+    a.group = tile_group(stream, policy);
     if (timer) timer->begin("seg_onesweep_pass", 2 * T.n_keys * (sizeof(U) + (KV ? sizeof(V) : 0)), stream);
     if (ident)
       hipLaunchKernelGGL((onesweep_kernel<U, V, KV, THREADS, ITEMS, uint32_t, true, true, Idx>),
@@ -1157,18 +1159,19 @@ static hipError_t radix_segments_t(U* keys, void* vals_v, const SegTiles& T, con
 
 template <typename U, int VB>
 static hipError_t radix_segments_i(U* keys, void* vals, const SegTiles& T, const XformParams& xp, unsigned char* ws,
-                                   const SegLayout& L, hipStream_t stream, LaunchTimer* timer) {
+                                   const SegLayout& L, hipStream_t stream, LaunchTimer* timer, int policy) {
   if (T.geo == 1) {
     if (T.n_total < (1ull << 32))
-      return radix_segments_t<U, VB, uint32_t, 1>(keys, vals, T, xp, ws, L, stream, timer);
-    return radix_segments_t<U, VB, uint64_t, 1>(keys, vals, T, xp, ws, L, stream, timer);
+      return radix_segments_t<U, VB, uint32_t, 1>(keys, vals, T, xp, ws, L, stream, timer, policy);
+    return radix_segments_t<U, VB, uint64_t, 1>(keys, vals, T, xp, ws, L, stream, timer, policy);
   }
-  if (T.n_total < (1ull << 32)) return radix_segments_t<U, VB, uint32_t, 0>(keys, vals, T, xp, ws, L, stream, timer);
-  return radix_segments_t<U, VB, uint64_t, 0>(keys, vals, T, xp, ws, L, stream, timer);
+  if (T.n_total < (1ull << 32)) return radix_segments_t<U, VB, uint32_t, 0>(keys, vals, T, xp, ws, L, stream, timer, policy);
+  return radix_segments_t<U, VB, uint64_t, 0>(keys, vals, T, xp, ws, L, stream, timer, policy);
 }
 
 hipError_t radix_sort_segments(int kb, int vb, void* keys, void* vals, const SegTiles& T, const XformParams& xf,
-                               void* workspace, size_t workspace_bytes, hipStream_t stream, LaunchTimer* timer) {
+                               void* workspace, size_t workspace_bytes, hipStream_t stream, LaunchTimer* timer,
+                               int policy) {
   if (T.tiles == 0) return hipSuccess;
   if (T.tiles >= (1ull << 31)) return hipErrorInvalidValue;
   const SegLayout L = seg_layout(kb, vb, T.n_total, T.tiles, T.nslots);
@@ -1176,14 +1179,14 @@ hipError_t radix_sort_segments(int kb, int vb, void* keys, void* vals, const Seg
   unsigned char* ws = reinterpret_cast<unsigned char*>(workspace);
   if (kb == 4) {
     uint32_t* k = reinterpret_cast<uint32_t*>(keys);
-    if (vb == 0) return radix_segments_i<uint32_t, 0>(k, vals, T, xf, ws, L, stream, timer);
-    if (vb == 4) return radix_segments_i<uint32_t, 4>(k, vals, T, xf, ws, L, stream, timer);
-    if (vb == 8) return radix_segments_i<uint32_t, 8>(k, vals, T, xf, ws, L, stream, timer);
+    if (vb == 0) return radix_segments_i<uint32_t, 0>(k, vals, T, xf, ws, L, stream, timer, policy);
+    if (vb == 4) return radix_segments_i<uint32_t, 4>(k, vals, T, xf, ws, L, stream, timer, policy);
+    if (vb == 8) return radix_segments_i<uint32_t, 8>(k, vals, T, xf, ws, L, stream, timer, policy);
   } else if (kb == 8) {
     uint64_t* k = reinterpret_cast<uint64_t*>(keys);
-    if (vb == 0) return radix_segments_i<uint64_t, 0>(k, vals, T, xf, ws, L, stream, timer);
-    if (vb == 4) return radix_segments_i<uint64_t, 4>(k, vals, T, xf, ws, L, stream, timer);
-    if (vb == 8) return radix_segments_i<uint64_t, 8>(k, vals, T, xf, ws, L, stream, timer);
+    if (vb == 0) return radix_segments_i<uint64_t, 0>(k, vals, T, xf, ws, L, stream, timer, policy);
+    if (vb == 4) return radix_segments_i<uint64_t, 4>(k, vals, T, xf, ws, L, stream, timer, policy);
+    if (vb == 8) return radix_segments_i<uint64_t, 8>(k, vals, T, xf, ws, L, stream, timer, policy);
   }
   return hipErrorInvalidValue;
 }
@@ -1281,7 +1284,7 @@ template <typename U, int VB, typename S>
 static hipError_t multisplit_t(const U* keys, const void* vals, uint64_t n, const U* d_spl, uint32_t nspl, int upper,
                                const XformParams& xp, U* out_keys, void* out_vals, unsigned char* ws,
                                const SplitLayout& L, uint64_t* d_bucket_base, hipStream_t stream,
-                               LaunchTimer* timer) {
+                               LaunchTimer* timer, int policy) {
   using V = typename ValT<VB>::T;
   using Idx = typename std::conditional<sizeof(S) == 4, uint32_t, uint64_t>::type;
   constexpr bool KV = VB != 0;
@@ -1322,7 +1325,7 @@ static hipError_t multisplit_t(const U* keys, const void* vals, uint64_t n, cons
   a.nspl = nspl;
   a.upper = upper;
   a.depth = depth;
-  a.group = tile_group(stream);
+  a.group = tile_group(stream, policy);
   if (timer) timer->begin("multisplit_pass", 2 * n * (sizeof(U) + (KV ? sizeof(V) : 0)), stream);
   hipLaunchKernelGGL((onesweep_kernel<U, V, KV, THREADS, ITEMS, S, false, false, Idx, true>), dim3((uint32_t)L.tiles),
                      dim3(THREADS), 0, stream, a);
@@ -1334,20 +1337,20 @@ template <typename U, int VB>
 static hipError_t multisplit_s(const U* keys, const void* vals, uint64_t n, const void* d_spl, uint32_t nspl,
                                int upper, const XformParams& xp, void* out_keys, void* out_vals, unsigned char* ws,
                                const SplitLayout& L, uint64_t* d_bucket_base, hipStream_t stream,
-                               LaunchTimer* timer) {
+                               LaunchTimer* timer, int policy) {
   const U* spl = reinterpret_cast<const U*>(d_spl);
   U* ok = reinterpret_cast<U*>(out_keys);
   if (L.status_word == 4)
     return multisplit_t<U, VB, uint32_t>(keys, vals, n, spl, nspl, upper, xp, ok, out_vals, ws, L, d_bucket_base,
-                                         stream, timer);
+                                         stream, timer, policy);
   return multisplit_t<U, VB, uint64_t>(keys, vals, n, spl, nspl, upper, xp, ok, out_vals, ws, L, d_bucket_base,
-                                       stream, timer);
+                                       stream, timer, policy);
 }
 
 hipError_t multisplit(int kb, int vb, const void* keys, const void* vals, uint64_t n, const void* d_splitters,
                       uint32_t nspl, int upper, const XformParams& xf, void* out_keys, void* out_vals,
                       uint64_t* d_bucket_base, void* workspace, size_t workspace_bytes, hipStream_t stream,
-                      LaunchTimer* timer) {
+                      LaunchTimer* timer, int policy) {
   if (n == 0) return hipSuccess;
   if (nspl > 255) return hipErrorInvalidValue;
   const SplitLayout L = split_layout(kb, vb, n);
@@ -1357,24 +1360,24 @@ hipError_t multisplit(int kb, int vb, const void* keys, const void* vals, uint64
     const uint32_t* k = reinterpret_cast<const uint32_t*>(keys);
     if (vb == 0)
       return multisplit_s<uint32_t, 0>(k, vals, n, d_splitters, nspl, upper, xf, out_keys, out_vals, ws, L,
-                                       d_bucket_base, stream, timer);
+                                       d_bucket_base, stream, timer, policy);
     if (vb == 4)
       return multisplit_s<uint32_t, 4>(k, vals, n, d_splitters, nspl, upper, xf, out_keys, out_vals, ws, L,
-                                       d_bucket_base, stream, timer);
+                                       d_bucket_base, stream, timer, policy);
     if (vb == 8)
       return multisplit_s<uint32_t, 8>(k, vals, n, d_splitters, nspl, upper, xf, out_keys, out_vals, ws, L,
-                                       d_bucket_base, stream, timer);
+                                       d_bucket_base, stream, timer, policy);
   } else if (kb == 8) {
     const uint64_t* k = reinterpret_cast<const uint64_t*>(keys);
     if (vb == 0)
       return multisplit_s<uint64_t, 0>(k, vals, n, d_splitters, nspl, upper, xf, out_keys, out_vals, ws, L,
-                                       d_bucket_base, stream, timer);
+                                       d_bucket_base, stream, timer, policy);
     if (vb == 4)
       return multisplit_s<uint64_t, 4>(k, vals, n, d_splitters, nspl, upper, xf, out_keys, out_vals, ws, L,
-                                       d_bucket_base, stream, timer);
+                                       d_bucket_base, stream, timer, policy);
     if (vb == 8)
       return multisplit_s<uint64_t, 8>(k, vals, n, d_splitters, nspl, upper, xf, out_keys, out_vals, ws, L,
-                                       d_bucket_base, stream, timer);
+                                       d_bucket_base, stream, timer, policy);
   }
   return hipErrorInvalidValue;
 }

@@ -127,7 +127,8 @@ bool seg_plan_empty(const SegPlanHost& P) {
 // Kernel launches only (no copies from host memory, no allocation, no
 // synchronisation): dm is the device copy of P.meta, ws holds P.ws_bytes.
 hipError_t launch_seg_plan(const SegPlanHost& P, const unsigned char* dm, void* ws, void* d_keys, void* d_vals,
-                           const XformParams& xf, hipStream_t stream, LaunchTimer* timer) {
+                           const XformParams& xf, hipStream_t stream, LaunchTimer* timer,
+                           int tile_policy = kTilesByStream) {
   hipError_t e = hipSuccess;
   if (!P.small_ids.empty()) {
     e = block_sort_segments(P.kb, P.vb, d_keys, d_vals, reinterpret_cast<const uint64_t*>(dm + P.off_offsets),
@@ -148,7 +149,7 @@ hipError_t launch_seg_plan(const SegPlanHost& P, const unsigned char* dm, void* 
     T.n_multi = (uint32_t)P.multi_ids[g].size();
     T.n_total = P.n_total;
     T.n_keys = G.n_keys;
-    e = radix_sort_segments(P.kb, P.vb, d_keys, d_vals, T, xf, ws, P.ws_bytes, stream, timer);
+    e = radix_sort_segments(P.kb, P.vb, d_keys, d_vals, T, xf, ws, P.ws_bytes, stream, timer, tile_policy);
     if (e != hipSuccess) return e;
   }
   return hipSuccess;
@@ -170,7 +171,7 @@ gs_status sort_segments_async(gs_handle* h, gs_dtype kdt, void* d_keys, gs_dtype
   if (e == hipSuccess && P.ws_bytes) e = h->arena.get(Arena::kSegWork, P.ws_bytes, h->stream, &ws);
   if (e != hipSuccess) return from_hip(e, "segment maps and workspace");
   e = launch_seg_plan(P, reinterpret_cast<unsigned char*>(dmeta), ws, d_keys, d_vals,
-                      make_xform((int)kdt, descending != 0), h->stream, h->timer());
+                      make_xform((int)kdt, descending != 0), h->stream, h->timer(), h->tile_policy);
   return from_hip(e, "segmented sort");
 }
 
@@ -225,6 +226,11 @@ struct gs_seg_plan {
   void* graph_keys = nullptr;
   void* graph_vals = nullptr;
   hipStream_t stream() const { return h->stream ? h->stream : own_stream; }
+  // The plan's own stream stands in for the handle's null stream (which a
+  // graph cannot capture), so it keeps the null stream's grouped hand-out.
+  int tile_policy() const {
+    return (h->tile_policy == kTilesByStream && h->stream == nullptr) ? (int)kTilesGrouped : h->tile_policy;
+  }
   void drop_graph() {
     if (exec) hipGraphExecDestroy(exec);
     if (graph) hipGraphDestroy(graph);
@@ -293,7 +299,7 @@ gs_status gs_seg_plan_run(gs_seg_plan* p, void* d_keys, void* d_vals) {
   if (p->host.n_total && (!d_keys || (p->vdt != GS_NONE && !d_vals))) return fail(GS_INVALID_ARG, "null pointer");
   DeviceScope ds(p->h->device);
   return from_hip(launch_seg_plan(p->host, reinterpret_cast<unsigned char*>(p->d_meta), p->d_ws, d_keys, d_vals, p->xf,
-                                  p->stream(), p->h->timer()),
+                                  p->stream(), p->h->timer(), p->tile_policy()),
                   "planned segmented sort");
 }
 
@@ -307,7 +313,7 @@ gs_status gs_seg_plan_launch_graph(gs_seg_plan* p, void* d_keys, void* d_vals) {
     hipError_t e = hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal);
     if (e != hipSuccess) return from_hip(e, "begin graph capture");
     const hipError_t le = launch_seg_plan(p->host, reinterpret_cast<unsigned char*>(p->d_meta), p->d_ws, d_keys,
-                                          d_vals, p->xf, st, nullptr);
+                                          d_vals, p->xf, st, nullptr, p->tile_policy());
     e = hipStreamEndCapture(st, &p->graph);
     if (le != hipSuccess) return from_hip(le, "capture planned sort");
     if (e == hipSuccess) e = hipGraphInstantiate(&p->exec, p->graph, nullptr, nullptr, 0);
@@ -421,7 +427,8 @@ extern "C" gs_status gs_autotune_segments(gs_handle* h, gs_dtype kdt, gs_dtype v
       if (e == hipSuccess && vb) e = hipMemcpyAsync(v1, v0, n * vb, hipMemcpyDeviceToDevice, h->stream);
       if (e == hipSuccess) e = hipEventRecord(ev0, h->stream);
       if (e == hipSuccess)
-        e = launch_seg_plan(P, reinterpret_cast<unsigned char*>(dmeta), ws, k1, v1, xf, h->stream, nullptr);
+        e = launch_seg_plan(P, reinterpret_cast<unsigned char*>(dmeta), ws, k1, v1, xf, h->stream, nullptr,
+                            h->tile_policy);
       if (e == hipSuccess) e = hipEventRecord(ev1, h->stream);
       if (e == hipSuccess) e = hipEventSynchronize(ev1);
       float t = 0.f;

@@ -116,6 +116,11 @@ struct gs_server {
   std::chrono::steady_clock::time_point t0;
   hipEvent_t ev0 = nullptr;  // device-clock origin of dev_start_us / dev_done_us
   uint64_t next_batch = 1;
+  // The one slot whose sorts may hand out tiles grouped by XCD (-1: none).
+  // Grouped hand-out relies on the pass getting enough resident workgroups,
+  // which holds when no other grouped pass runs beside it; every other slot
+  // uses the arrival-order ticket, which progresses whatever else runs.
+  int grouped_slot = -1;
 
   size_t pooled_bytes() const {
     size_t b = 0;
@@ -290,6 +295,11 @@ bool gs_server::admit_one(Slot& slot, std::unique_lock<std::mutex>& lk) {
   }
   slot.busy = true;
   slot.reqs = ids;
+  {
+    const int me = (int)(&slot - slots.data());
+    if (grouped_slot < 0) grouped_slot = me;
+    slot.h->tile_policy = grouped_slot == me ? kTilesGrouped : kTilesArrival;
+  }
   std::vector<Request> copies;
   for (uint64_t id : ids) copies.push_back(reqs[id]);
   st.batches++;
@@ -309,6 +319,7 @@ bool gs_server::admit_one(Slot& slot, std::unique_lock<std::mutex>& lk) {
     }
     st.failed += ids.size();
     st.completed += ids.size();
+    if (grouped_slot == (int)(&slot - slots.data())) grouped_slot = -1;
     slot.busy = false;
     cv_done.notify_all();
   }
@@ -344,6 +355,7 @@ void gs_server::run() {
       }
       st.completed += s.reqs.size();
       if (q != hipSuccess) st.failed += s.reqs.size();
+      if (grouped_slot == (int)(&s - slots.data())) grouped_slot = -1;
       s.busy = false;
       s.reqs.clear();
       progress = true;

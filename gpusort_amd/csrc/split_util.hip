@@ -9,6 +9,8 @@
 // payload the runs are compared as (key, payload) pairs, the order the
 // reference's rounds use (std::pair under the lifted comparator,
 // include/common.h:156-164).
+#include <type_traits>
+
 #include "gs_device.h"
 #include "gs_internal.h"
 
@@ -124,14 +126,40 @@ hipError_t gather_elems(int bytes, const void* src, const uint64_t* d_idx, void*
 }
 
 // Flag set when some element is ordered before its predecessor under the
-// transform (a run that is not non-decreasing in the requested order).
+// transform (a run that is not non-decreasing in the requested order).  Each
+// lane checks one 16-byte vector and the element before it.
 template <typename T>
 __global__ __launch_bounds__(256) void unsorted_flag_kernel(const T* __restrict__ a, uint64_t n, Xform<T> xf,
                                                             uint32_t* __restrict__ flag) {
+  constexpr uint32_t VN = 16 / sizeof(T);
+  using VT = typename std::conditional<sizeof(T) == 4, uint4, ulonglong2>::type;
+  const uint64_t nvec = n / VN;
   const uint64_t stride = (uint64_t)gridDim.x * 256;
+  const VT* av = reinterpret_cast<const VT*>(a);
   bool bad = false;
-  for (uint64_t i = (uint64_t)blockIdx.x * 256 + threadIdx.x + 1; i < n; i += stride)
-    bad |= to_ordered(a[i], xf) < to_ordered(a[i - 1], xf);
+  for (uint64_t v = (uint64_t)blockIdx.x * 256 + threadIdx.x; v < nvec; v += stride) {
+    const VT x = av[v];
+    T e[VN];
+    if constexpr (VN == 4) {
+      e[0] = x.x;
+      e[1] = x.y;
+      e[2] = x.z;
+      e[3] = x.w;
+    } else {
+      e[0] = x.x;
+      e[1] = x.y;
+    }
+    T prev = v ? to_ordered(a[v * VN - 1], xf) : T(0);
+#pragma unroll
+    for (uint32_t j = 0; j < VN; ++j) {
+      const T cur = to_ordered(e[j], xf);
+      bad |= cur < prev;
+      prev = cur;
+    }
+  }
+  // Elements past the last whole vector.
+  for (uint64_t i = nvec * VN + (uint64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += stride)
+    if (i > 0) bad |= to_ordered(a[i], xf) < to_ordered(a[i - 1], xf);
   if (__syncthreads_or(bad) && threadIdx.x == 0) atomicOr(flag, 1u);
 }
 

@@ -61,6 +61,8 @@ struct Pipe {
   uint64_t n;
   // host-side aux staging (device copies live in the aux arena slot)
   std::vector<char> stage;
+  // host sources of queued uploads, kept until the call's final wait
+  std::vector<std::vector<unsigned char>> keep;
 };
 
 bool is_pow2(uint64_t x) { return x && !(x & (x - 1)); }
@@ -94,8 +96,9 @@ gs_status sort_runs(Pipe& P, char* k, char* v, const std::vector<uint64_t>& off,
                        P.desc);
       if (st != GS_OK) return st;
     }
-    // `meta` is the source of an upload queued above.
-    return from_hip(hipStreamSynchronize(P.h->stream), "local sort");
+    // `meta` is the source of an upload queued above: kept until the end.
+    P.keep.push_back(std::move(meta));
+    return GS_OK;
   };
   if (!P.kv) return pass(P.kdt, k, GS_NONE, nullptr, P.kb, 0);
   if (lex_values) {
