@@ -379,7 +379,9 @@ def run_multisplit(h, g, torch, n: int, nspl: int = 63, steps: int = 5):
     torch.cuda.empty_cache()
     return {"n": n, "buckets": nspl + 1, "ms": round((t1 - t0) * 1e3 / steps, 4),
             "pass_ms": round(avg, 4), "pass_GB/s": round(gbs, 1), "pass_frac": round(gbs / HBM_PEAK_GBS, 4),
+            "pass_traffic": traffic_from_profile(n, "pmc_multisplit_kv_u32_2p28.txt"),
             "hist_ms": round(hs["total_ms"] / hs["launches"], 4) if hs else None,
+            "hist_frac": round(hs["bytes"] / hs["total_ms"] * 1e3 / 1e9 / HBM_PEAK_GBS, 4) if hs else None,
             "verified": ok_perm and in_range and stable}
 
 
@@ -433,7 +435,9 @@ def run_global_index(h, g, torch, n: int, steps: int = 5):
     if gather_ms:
         gbs = gather_bytes / (gather_ms * 1e-3) / 1e9
         out.update({"gathers_ms": round(gather_ms, 4), "gather_GB/s": round(gbs, 1),
-                    "gather_frac": round(gbs / HBM_PEAK_GBS, 4)})
+                    "gather_frac": round(gbs / HBM_PEAK_GBS, 4), "gather_algorithmic_bytes": gather_bytes,
+                    "gather_traffic": traffic_from_profile(n, "pmc_gather_global_index_2p27.txt",
+                                                           "traffic_bytes_three_gathers")})
     return out
 
 
@@ -501,7 +505,12 @@ def run_c3(h, g, torch, with_cpu: bool, steps: int = 3):
     out = {"p": p, "k_way": 8, "samples_per_rank": 32, "n": n, "rounds": rep["rounds"],
            "gpu_ms": round(ms, 3), "gpu_gkeys_s": round(n / (ms * 1e-3) / 1e9, 3),
            "max_rank_error": rep["max_rank_error"],
+           "splitter_iterations": rep.get("splitter_iterations"),
            "kernels_ms": {name: round(st["total_ms"] / steps, 3) for name, st in prof.items()}}
+    mroof = roofline_of(prof, "kway_merge_kernel")
+    if mroof is not None:
+        mroof["traffic"] = traffic_from_profile(n, "pmc_kway_merge_vrank_hyksort_2p27_u64_p64.txt")
+    out["kway_merge_roofline"] = mroof
     if with_cpu:
         import oracle
 
@@ -685,16 +694,17 @@ def cpu_baseline(n_sample: int, reps: int = 3):
     }
 
 
-def traffic_from_profile(n: int):
-    """HBM bytes per onesweep launch from the committed FETCH_SIZE / WRITE_SIZE
-    summary (rocprofv3 --pmc, scaled by the plain-copy calibration), when it
-    was taken on this workload size."""
-    path = os.path.join(ROOT, "profiles", "pmc_onesweep_kv_u32_2p28.txt")
+def traffic_from_profile(n: int, fname: str = "pmc_onesweep_kv_u32_2p28.txt",
+                         field: str = "traffic_bytes_per_launch"):
+    """HBM bytes per launch of a kernel from its committed FETCH_SIZE /
+    WRITE_SIZE summary under profiles/ (rocprofv3 --pmc, scaled by the
+    plain-copy calibration), when it was taken on this workload size."""
+    path = os.path.join(ROOT, "profiles", fname)
     try:
         text = open(path).read()
     except OSError:
         return None
-    m = re.search(r"^traffic_bytes_per_launch\s*=\s*(\d+)", text, re.M)
+    m = re.search(r"^" + re.escape(field) + r"\s*=\s*(\d+)", text, re.M)
     nk = re.search(r"^n_keys\s*=\s*(\d+)", text, re.M)
     if not m or not nk or int(nk.group(1)) != n:
         return None
@@ -785,8 +795,12 @@ def main() -> int:
         if args.only in ("keys", "kv"):
             r = run_workload(h, g, torch, "uniform", n, args.steps, args.warmup, payload=args.only == "kv",
                              profile=True)
+            roof1 = roofline_of(r["profile"])
+            if roof1 is not None:
+                roof1["traffic"] = traffic_from_profile(
+                    n, "pmc_onesweep_kv_u32_2p28.txt" if args.only == "kv" else "pmc_onesweep_keys_u32_2p28.txt")
             r = {"gkeys_s": round(r["gkeys_s"], 3), "ms": round(r["ms_per_sort"], 4), "verified": r["verified"],
-                 "roofline": roofline_of(r["profile"]), "profile": r["profile"]}
+                 "roofline": roof1, "profile": r["profile"]}
         elif args.only == "vrank":
             r = run_c3(h, g, torch, not args.no_cpu)
         elif args.only == "batched":
@@ -809,8 +823,11 @@ def main() -> int:
     if not args.no_extra:
         log("[bench] keys-only u32 uniform")
         r = run_workload(h, g, torch, "uniform", n, 3, 1, payload=False, profile=True)
+        kroof = roofline_of(r["profile"])
+        if kroof is not None:
+            kroof["traffic"] = traffic_from_profile(n, "pmc_onesweep_keys_u32_2p28.txt")
         extra["keys_only_u32_uniform"] = {"gkeys_s": round(r["gkeys_s"], 3), "ms": round(r["ms_per_sort"], 3),
-                                          "verified": r["verified"], "roofline": roofline_of(r["profile"])}
+                                          "verified": r["verified"], "roofline": kroof}
         log("[bench] KV u32 uniform on a side stream (tiles handed out in arrival order)")
         side = torch.cuda.Stream()
         h.set_stream(side.cuda_stream)

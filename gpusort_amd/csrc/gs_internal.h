@@ -228,8 +228,12 @@ struct KwayDevice {
 // Merges every destination from (sk, sv) into (dk, dv); `sizes` bounds the
 // launches.  lex_vxf: when given (and there is a payload) elements compare as
 // (key, payload) pairs.
+// With a timer: one record for the boundary kernels ("kway_setup") and one
+// for the merge kernel ("kway_merge_kernel", `elems` elements read and
+// written).
 hipError_t kway_merge(int kb, int vb, const void* sk, const void* sv, void* dk, void* dv, const KwaySizes& sizes,
-                      const KwayDevice& dev, const XformParams& xf, const XformParams* lex_vxf, hipStream_t stream);
+                      const KwayDevice& dev, const XformParams& xf, const XformParams* lex_vxf, hipStream_t stream,
+                      LaunchTimer* timer = nullptr, uint64_t elems = 0);
 
 // ---------------------------------------------------------------- HykSort rounds
 // HykSort's rounds over p virtual ranks with every step on the device

@@ -753,10 +753,9 @@ hipError_t hyk_rounds_t(HykRun* run, const uint64_t* h_off, char* ws, const HykL
       const KwaySizes ub = kway_upper_bounds(run->n, p * chunks);
       const KwayDevice dev{dests, totals, ws + L.kscratch};
       const int c2 = run->cur;
-      if (timer) timer->begin("kway_merge", 2 * run->n * (sizeof(U) + VB), stream);
       e = kway_merge((int)sizeof(U), VB, run->kbuf[c2], VB ? run->vbuf[c2] : nullptr, run->kbuf[c2 ^ 1],
-                     VB ? run->vbuf[c2 ^ 1] : nullptr, ub, dev, run->kx, LEX ? &run->vx : nullptr, stream);
-      if (timer) timer->end(stream);
+                     VB ? run->vbuf[c2 ^ 1] : nullptr, ub, dev, run->kx, LEX ? &run->vx : nullptr, stream, timer,
+                     run->n);
       if (e != hipSuccess) return e;
       run->cur ^= 1;
     }

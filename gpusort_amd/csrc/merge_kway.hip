@@ -43,6 +43,10 @@ constexpr uint32_t kKwayCap = (kSamplesPerPart + kKwayMax) * kSampleStride;
 constexpr int kKwayThreads = 256;
 constexpr int kKwayItems = 13;
 static_assert(kKwayItems * kKwayThreads >= (int)kKwayCap, "tile covers the cap");
+// A merge level gives each group ceil(len / kKwayItems) threads: at most
+// cap / items + (groups at the first level) of them.
+static_assert((kKwayCap + kKwayItems - 1) / kKwayItems + kKwayMax / 2 <= (uint32_t)kKwayThreads,
+              "every group of a level has its threads");
 
 // Element of the merged order: ordered key and, with LEX, ordered payload.
 template <typename U, typename V, bool LEX>
@@ -247,6 +251,62 @@ __global__ __launch_bounds__(256) void kway_refine_kernel(const U* __restrict__ 
   out[r] = lo;
 }
 
+// Everything a merge workgroup needs of its partition, in one 128-byte
+// record: the source element of each piece's slot 0 (slot e of piece r reads
+// src[r] + e), the slot where each piece starts (start[m..] = total), the
+// output offset and the number of runs.  A partition past the pass's count
+// has total 0.
+struct alignas(16) KwayPart {
+  uint64_t src[kKwayMax];
+  uint64_t out;
+  uint32_t start[kKwayMax + 1];
+  uint32_t m;
+  uint32_t pad[4];
+};
+static_assert(sizeof(KwayPart) == 128, "one 128-byte record per partition");
+
+// One lane per partition (of the upper bound the grid is sized by): its
+// destination (binary search over the destinations' first partitions), its
+// boundaries and the record above.  Done here, with every partition's
+// search in flight at once, rather than by one thread of each merge
+// workgroup while the others wait.
+__global__ __launch_bounds__(256) void kway_part_kernel(const KwayDest* __restrict__ dests,
+                                                        const KwayTotals* __restrict__ tot,
+                                                        const uint64_t* __restrict__ bounds,
+                                                        KwayPart* __restrict__ parts, uint32_t nparts_ub) {
+  const uint32_t x = blockIdx.x * 256 + threadIdx.x;
+  if (x >= nparts_ub) return;
+  KwayPart P;
+  for (uint32_t r = 0; r < kKwayMax; ++r) {
+    P.src[r] = 0;
+    P.start[r] = 0;
+  }
+  P.start[kKwayMax] = 0;
+  P.out = 0;
+  P.m = 0;
+  for (int i = 0; i < 4; ++i) P.pad[i] = 0;
+  if (x < tot->nparts) {
+    const uint32_t d = find_dest(tot->ndests, x, [&](uint32_t y) { return dests[y].part_base; });
+    const KwayDest& D = dests[d];
+    const uint64_t q = (uint64_t)D.bnd_base + (x - D.part_base);
+    uint32_t e = 0;
+    uint64_t out = D.out_off;
+    for (uint32_t r = 0; r < kKwayMax; ++r) {
+      const uint64_t lo = bounds[q * kKwayMax + r];
+      const uint64_t hi = bounds[(q + 1) * kKwayMax + r];
+      const uint32_t len = r < D.m ? (uint32_t)(hi - lo) : 0u;
+      P.src[r] = D.run_off[r] + lo - e;
+      P.start[r] = e;
+      e += len;
+      out += r < D.m ? lo : 0;
+    }
+    P.start[kKwayMax] = e;
+    P.out = out;
+    P.m = D.m;
+  }
+  parts[x] = P;
+}
+
 // Slots of piece group [r0, r1) of a partition: pieces r0..r1-1 (clamped
 // to m) are contiguous in the staging buffer.
 __device__ inline uint32_t group_start(const uint32_t* s_start, uint32_t r, uint32_t m) {
@@ -256,46 +316,29 @@ __device__ inline uint32_t group_start(const uint32_t* s_start, uint32_t r, uint
 template <typename U, typename V, bool KV, bool LEX>
 __global__ __launch_bounds__(kKwayThreads) void kway_merge_kernel(const U* __restrict__ sk, const V* __restrict__ sv,
                                                                   U* __restrict__ dk, V* __restrict__ dv,
-                                                                  const KwayDest* __restrict__ dests,
-                                                                  const KwayTotals* __restrict__ tot,
-                                                                  const uint64_t* __restrict__ bounds, Xform<U> xf,
+                                                                  const KwayPart* __restrict__ parts, Xform<U> xf,
                                                                   Xform<V> vxf) {
   constexpr int VT = kKwayItems;
   // Elements (ordered key, payload: its ordered image with LEX) of the
   // partition, merged level by level in place: every level's outputs are
   // gathered in registers before any is written back.
-  __shared__ U s_k[kKwayCap];
-  __shared__ V s_v[KV ? kKwayCap : 1];
-  __shared__ uint64_t s_src[kKwayMax];  // source element of each piece's slot 0
-  __shared__ uint32_t s_start[kKwayMax + 1];
-  __shared__ uint64_t s_out;
-  __shared__ uint32_t s_m;
+  // VT slack slots past the cap: a thread's steps past its group's end
+  // read at most VT - 1 slots beyond it.
+  __shared__ U s_k[kKwayCap + VT];
+  __shared__ V s_v[KV ? kKwayCap + VT : 1];
+  __shared__ KwayPart s_part;
 
   const uint32_t tid = threadIdx.x;
-  if (blockIdx.x >= tot->nparts) return;  // grid sized by an upper bound
-  if (tid == 0) {
-    const uint32_t d = find_dest(tot->ndests, blockIdx.x, [&](uint32_t x) { return dests[x].part_base; });
-    const KwayDest& D = dests[d];
-    const uint64_t q = (uint64_t)D.bnd_base + (blockIdx.x - D.part_base);
-    uint32_t e = 0;
-    uint64_t out = D.out_off;
-    for (uint32_t r = 0; r < kKwayMax; ++r) {
-      const uint64_t lo = bounds[q * kKwayMax + r];
-      const uint64_t hi = bounds[(q + 1) * kKwayMax + r];
-      const uint32_t len = r < D.m ? (uint32_t)(hi - lo) : 0u;
-      s_src[r] = D.run_off[r] + lo - e;  // slot e0 + x of piece r reads s_src[r] + e0 + x
-      s_start[r] = e;
-      e += len;
-      out += r < D.m ? lo : 0;
-    }
-    s_start[kKwayMax] = e;
-    s_out = out;
-    s_m = D.m;
-  }
+  // The partition's record: 32 lanes, one dword each.
+  if (tid < sizeof(KwayPart) / 4)
+    reinterpret_cast<uint32_t*>(&s_part)[tid] = reinterpret_cast<const uint32_t*>(parts + blockIdx.x)[tid];
   __syncthreads();
+  const uint32_t* const s_start = s_part.start;
+  const uint64_t* const s_src = s_part.src;
   const uint32_t total = s_start[kKwayMax];
-  const uint32_t m = s_m;
-  const uint64_t out_base = s_out;
+  if (total == 0) return;  // past the pass's partitions (grid sized by an upper bound)
+  const uint32_t m = s_part.m;
+  const uint64_t out_base = s_part.out;
 
   // Stage the pieces in list order (coalesced within each piece; every
   // thread's loads are issued before any is used).
@@ -327,22 +370,33 @@ __global__ __launch_bounds__(kKwayThreads) void kway_merge_kernel(const U* __res
 
   // Merge tree over the pieces: at level L group j (pieces [j*2^L,
   // (j+1)*2^L)) is the merge of its two halves, the earlier half winning
-  // ties.  Thread t produces positions [t*VT, t*VT + VT) of the level: one
-  // merge-path search for its first position, then a serial merge that
-  // reads one element per output; entering the next group (at its first
-  // position, where the path starts at both halves' beginnings) needs no
-  // search.
-  const uint32_t o0 = tid * VT;
+  // ties.  Each group takes ceil(len / VT) threads in group order; a
+  // thread produces VT consecutive positions of its group (fewer at the
+  // group's end) with one merge-path search for its first position, then a
+  // serial merge of VT steps with no branch: a step past the group's end
+  // reads within the slack slots, and its output is not written back.
   for (uint32_t width = 1; width < m; width <<= 1) {
     U rk[VT];
-    V rv[KV ? VT : 1];
-    if (o0 < total) {
-      uint32_t g0 = 0;
-      while (group_start(s_start, g0 + 2 * width, m) <= o0) g0 += 2 * width;
+    V rv[VT];  // unused (and dropped) without payloads
+    uint32_t g0 = 0, t0 = 0;
+    bool mine = false;
+    for (uint32_t g = 0; g < m; g += 2 * width) {
+      const uint32_t len = group_start(s_start, g + 2 * width, m) - group_start(s_start, g, m);
+      const uint32_t nt = (len + VT - 1) / VT;
+      if (!mine && tid < t0 + nt) {
+        mine = true;
+        g0 = g;
+      }
+      if (!mine) t0 += nt;
+    }
+    uint32_t o0 = 0, cnt = 0;
+    if (mine) {
       const uint32_t a0 = group_start(s_start, g0, m);
       const uint32_t b0 = group_start(s_start, g0 + width, m);
-      uint32_t b1 = group_start(s_start, g0 + 2 * width, m);
-      const uint32_t diag = o0 - a0;
+      const uint32_t b1 = group_start(s_start, g0 + 2 * width, m);
+      const uint32_t diag = (tid - t0) * VT;
+      o0 = a0 + diag;
+      cnt = b1 - o0 < (uint32_t)VT ? b1 - o0 : (uint32_t)VT;
       const uint32_t na = b0 - a0, nb = b1 - b0;
       uint32_t lo = diag > nb ? diag - nb : 0;
       {
@@ -363,69 +417,37 @@ __global__ __launch_bounds__(kKwayThreads) void kway_merge_kernel(const U* __res
             hi = mid;
         }
       }
-      // heads of the two halves: lo elements taken from the first
+      // Both heads are read at every step, at the sides' current indices,
+      // so no head is kept in registers; a side's index at or past its end
+      // reads a later slot (or a slack slot), ignored through the index.
       uint32_t ia = a0 + lo, ib = b0 + (diag - lo);
-      uint32_t a1 = b0;  // end of the first half
-      U ka = s_k[ia < a1 ? ia : 0], kb = s_k[ib < b1 ? ib : 0];
-      V va = V(0), vb = V(0);
-      if constexpr (KV) {
-        va = s_v[ia < a1 ? ia : 0];
-        vb = s_v[ib < b1 ? ib : 0];
-      }
-      // One LDS read per output, at the next element of the side just
-      // consumed (a side's head is ignored once the side is exhausted).
 #pragma unroll
       for (int j = 0; j < VT; ++j) {
-        const uint32_t o = o0 + (uint32_t)j;
-        if (o < total) {
-          if (o >= b1) {
-            // Next group (skipping empty ones): its path starts at the
-            // beginnings of its halves.  Rare: at most a few per thread.
-            while (o >= b1) {
-              g0 += 2 * width;
-              ia = b1;
-              a1 = group_start(s_start, g0 + width, m);
-              ib = a1;
-              b1 = group_start(s_start, g0 + 2 * width, m);
-            }
-            ka = s_k[ia < a1 ? ia : 0];
-            kb = s_k[ib < b1 ? ib : 0];
-            if constexpr (KV) {
-              va = s_v[ia < a1 ? ia : 0];
-              vb = s_v[ib < b1 ? ib : 0];
-            }
-          }
-          bool le;
-          if constexpr (LEX)
-            le = ka < kb || (ka == kb && va <= vb);
-          else
-            le = ka <= kb;
-          const bool take_a = ib >= b1 || (ia < a1 && le);
-          rk[j] = take_a ? ka : kb;
-          if constexpr (KV) rv[j] = take_a ? va : vb;
-          ia += take_a ? 1u : 0u;
-          ib += take_a ? 0u : 1u;
-          const uint32_t nx = take_a ? ia : ib;
-          const uint32_t lim = take_a ? a1 : b1;
-          const uint32_t at = nx < lim ? nx : 0u;
-          const U nk = s_k[at];
-          ka = take_a ? nk : ka;
-          kb = take_a ? kb : nk;
-          if constexpr (KV) {
-            const V nv = s_v[at];
-            va = take_a ? nv : va;
-            vb = take_a ? vb : nv;
-          }
+        const U ka = s_k[ia], kb = s_k[ib];
+        V va = V(0), vb = V(0);
+        if constexpr (KV) {
+          va = s_v[ia];
+          vb = s_v[ib];
         }
+        // first half on ties; bitwise operators keep the step branch-free
+        bool le;
+        if constexpr (LEX)
+          le = (ka < kb) | ((ka == kb) & (va <= vb));
+        else
+          le = ka <= kb;
+        const bool take_a = (ib >= b1) | ((ia < b0) & le);
+        rk[j] = take_a ? ka : kb;
+        if constexpr (KV) rv[j] = take_a ? va : vb;
+        ia += take_a ? 1u : 0u;
+        ib += take_a ? 0u : 1u;
       }
     }
     __syncthreads();
 #pragma unroll
     for (int j = 0; j < VT; ++j) {
-      const uint32_t o = o0 + (uint32_t)j;
-      if (o < total) {
-        s_k[o] = rk[j];
-        if constexpr (KV) s_v[o] = rv[j];
+      if ((uint32_t)j < cnt) {
+        s_k[o0 + j] = rk[j];
+        if constexpr (KV) s_v[o0 + j] = rv[j];
       }
     }
     __syncthreads();
@@ -448,7 +470,7 @@ struct KValT<8> {
 // Device scratch layout: boundaries, boundary runs, compact sample keys and
 // payloads, for the pass's upper bounds.
 struct KwayScratch {
-  size_t bounds, bound_run, samp_k, samp_v, total;
+  size_t bounds, bound_run, samp_k, samp_v, parts, total;
 };
 
 KwayScratch kway_scratch(const KwaySizes& B, int kb, int vb) {
@@ -462,13 +484,16 @@ KwayScratch kway_scratch(const KwaySizes& B, int kb, int vb) {
   o = align_up(o + (size_t)B.nsamples * kb, 256);
   L.samp_v = o;
   o = align_up(o + (size_t)B.nsamples * (vb ? vb : 4), 256);
+  L.parts = o;
+  o = align_up(o + (size_t)B.nparts * sizeof(KwayPart), 256);
   L.total = o;
   return L;
 }
 
 template <typename U, int VB, bool LEX>
 hipError_t kway_launch(const void* sk, const void* sv, void* dk, void* dv, const KwaySizes& B, const KwayDevice& dev,
-                       const XformParams& xp, const XformParams& vp, hipStream_t stream) {
+                       const XformParams& xp, const XformParams& vp, hipStream_t stream, LaunchTimer* timer,
+                       uint64_t elems) {
   using V = typename KValT<VB>::T;
   constexpr bool KV = VB != 0;
   const Xform<U> xf{(U)xp.float_mask, (U)xp.sign_flip, (U)xp.desc_mask};
@@ -479,6 +504,8 @@ hipError_t kway_launch(const void* sk, const void* sv, void* dk, void* dv, const
   uint32_t* bound_run = reinterpret_cast<uint32_t*>(ws + L.bound_run);
   U* samp_k = reinterpret_cast<U*>(ws + L.samp_k);
   V* samp_v = reinterpret_cast<V*>(ws + L.samp_v);
+  KwayPart* parts = reinterpret_cast<KwayPart*>(ws + L.parts);
+  if (timer) timer->begin("kway_setup", 0, stream);
   hipLaunchKernelGGL(kway_bounds_init_kernel, dim3((B.ndests + 255) / 256), dim3(256), 0, stream, dev.dests,
                      dev.totals, bounds);
   if (B.nsamples) {
@@ -492,9 +519,16 @@ hipError_t kway_launch(const void* sk, const void* sv, void* dk, void* dv, const
                        reinterpret_cast<const U*>(sk), reinterpret_cast<const V*>(sv), dev.dests, dev.totals, xf, vxf,
                        bounds, bound_run);
   }
+  hipLaunchKernelGGL(kway_part_kernel, dim3((B.nparts + 255) / 256), dim3(256), 0, stream, dev.dests, dev.totals,
+                     bounds, parts, B.nparts);
+  if (timer) {
+    timer->end(stream);
+    timer->begin("kway_merge_kernel", 2 * elems * (sizeof(U) + VB), stream);
+  }
   hipLaunchKernelGGL((kway_merge_kernel<U, V, KV, LEX>), dim3(B.nparts), dim3(kKwayThreads), 0, stream,
                      reinterpret_cast<const U*>(sk), reinterpret_cast<const V*>(sv), reinterpret_cast<U*>(dk),
-                     reinterpret_cast<V*>(dv), dev.dests, dev.totals, bounds, xf, vxf);
+                     reinterpret_cast<V*>(dv), parts, xf, vxf);
+  if (timer) timer->end(stream);
   return hipGetLastError();
 }
 
@@ -536,20 +570,21 @@ KwaySizes kway_upper_bounds(uint64_t n_elems, uint32_t ndests) {
 size_t kway_scratch_bytes(const KwaySizes& B, int kb, int vb) { return kway_scratch(B, kb, vb).total; }
 
 hipError_t kway_merge(int kb, int vb, const void* sk, const void* sv, void* dk, void* dv, const KwaySizes& B,
-                      const KwayDevice& dev, const XformParams& xf, const XformParams* lex_vxf, hipStream_t stream) {
+                      const KwayDevice& dev, const XformParams& xf, const XformParams* lex_vxf, hipStream_t stream,
+                      LaunchTimer* timer, uint64_t elems) {
   if (B.ndests == 0 || B.nparts == 0) return hipSuccess;
   const XformParams vz{0, 0, 0};
   const bool lex = lex_vxf != nullptr && vb != 0;
   const XformParams& vp = lex ? *lex_vxf : vz;
 #define GS_KWAY_CASE(UT, VBV)                                                                  \
-  return lex ? kway_launch<UT, VBV, true>(sk, sv, dk, dv, B, dev, xf, vp, stream)              \
-             : kway_launch<UT, VBV, false>(sk, sv, dk, dv, B, dev, xf, vp, stream)
+  return lex ? kway_launch<UT, VBV, true>(sk, sv, dk, dv, B, dev, xf, vp, stream, timer, elems)              \
+             : kway_launch<UT, VBV, false>(sk, sv, dk, dv, B, dev, xf, vp, stream, timer, elems)
   if (kb == 4) {
-    if (vb == 0) return kway_launch<uint32_t, 0, false>(sk, sv, dk, dv, B, dev, xf, vp, stream);
+    if (vb == 0) return kway_launch<uint32_t, 0, false>(sk, sv, dk, dv, B, dev, xf, vp, stream, timer, elems);
     if (vb == 4) GS_KWAY_CASE(uint32_t, 4);
     if (vb == 8) GS_KWAY_CASE(uint32_t, 8);
   } else if (kb == 8) {
-    if (vb == 0) return kway_launch<uint64_t, 0, false>(sk, sv, dk, dv, B, dev, xf, vp, stream);
+    if (vb == 0) return kway_launch<uint64_t, 0, false>(sk, sv, dk, dv, B, dev, xf, vp, stream, timer, elems);
     if (vb == 4) GS_KWAY_CASE(uint64_t, 4);
     if (vb == 8) GS_KWAY_CASE(uint64_t, 8);
   }
