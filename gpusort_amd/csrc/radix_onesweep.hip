@@ -184,6 +184,9 @@ __global__ __launch_bounds__(256) void radix_scan_kernel(const unsigned long lon
   }
 }
 
+template <typename U>
+struct SplitTab;
+
 template <typename U, typename V, typename S>
 struct OnesweepArgs {
   U* keys0;
@@ -208,12 +211,10 @@ struct OnesweepArgs {
   const uint64_t* seg_range;  // slot -> [start, end) element offsets, 2 words
   const uint32_t* seg_base;   // [slot][pass][256] digit starts within the segment
   int passes;                 // stride of seg_base
-  // Multisplit mode only: the "digit" is the bucket of the key among nspl
-  // sorted splitters (raw keys, device memory), found in an LDS search tree.
-  const U* split_keys;
-  uint32_t nspl;
+  // Multisplit mode only: the "digit" is the bucket of the key among the
+  // sorted splitters, read from their bucket table (copied into LDS).
+  const SplitTab<U>* split_tab;
   int upper;  // 1: bucket = #splitters < key; 0: #splitters <= key
-  int depth;  // levels of the splitter tree: 2^depth - 1 >= nspl
 };
 
 // Tile of the j-th workgroup of class c (blockIdx % 8) to take a ticket.
@@ -230,56 +231,140 @@ __device__ inline uint32_t class_tile(uint32_t c, uint32_t j, uint32_t G, uint32
   return full * 8u * G + (j - full * G) * 8u + c;
 }
 
-// Levels of the smallest perfect search tree that holds nspl splitters.
-inline int tree_depth(uint32_t nspl) {
-  int d = 0;
-  while (((1u << d) - 1) < nspl) ++d;
-  return d;
-}
+// Multisplit bucket of an ordered key image x among nspl sorted splitter
+// images s[0..nspl): #{j : s[j] < x} (upper) or #{j : s[j] <= x} (lower), a
+// non-decreasing function of x.  It is read from a table of kSplitTab
+// entries over the images: image x >= base falls in entry
+// min((x - base) >> shift, kSplitTab - 1), images below base in entry 0,
+// where base is the smallest splitter image and shift the least one that
+// maps the largest splitter inside the table.  An entry holds the buckets of
+// its first and last image, lo | hi << 8; the bucket of an image in it lies
+// in [lo, hi] and is found there by a binary search over splitters lo..hi-1,
+// which the entries that hold no splitter skip.  One LDS read per key
+// instead of a descent of up to eight levels.
+constexpr int kSplitTabBits = 12;
+constexpr uint32_t kSplitTab = 1u << kSplitTabBits;
 
-// Node of a perfect (2^depth - 1)-node search tree (heap order, root 1) that
-// holds the j-th smallest element: in-order rank r = j + 1 sits at level
-// depth - 1 - tz(r).
-__device__ inline uint32_t tree_node(uint32_t j, int depth) {
-  const uint32_t r = j + 1;
-  const uint32_t tz = __builtin_ctz(r);
-  return (1u << (depth - 1 - tz)) + (r >> (tz + 1));
-}
-
-// The tree is held in `ncopy` (a power of two, up to 32) interleaved copies:
-// node i of copy c at tree[i * ncopy + c], and lane l reads copy l % ncopy.
-// With 32 copies every lane of a wave reads its own bank at every level
-// whatever the path its key takes (one shared tree serialises lanes whose
-// paths hit different nodes of one bank).  Size: 2^depth * ncopy elements.
-constexpr uint32_t kTreeCopies = 32;
-
-// Bucket of an ordered key image: `depth` branch-free steps down the tree;
-// padding nodes hold the all-ones image, so the count is clamped to nspl.
 template <typename U>
-__device__ inline uint32_t tree_bucket(const U* tree, U img, int upper, uint32_t nspl, int depth, uint32_t ncopy,
-                                       uint32_t copy) {
-  uint32_t i = 1;
-#pragma unroll
-  for (int l = 0; l < 8; ++l) {
-    if (l >= depth) break;
-    const U t = tree[i * ncopy + copy];
-    i = 2 * i + (upper ? (img > t) : (img >= t));
+struct SplitTab {
+  U base;
+  uint32_t shift;
+  uint32_t nspl;
+  U img[kRadix];                // splitter images, ascending; all ones past nspl
+  uint16_t entry[kSplitTab];    // lo | hi << 8
+};
+
+template <typename U>
+__device__ inline uint32_t count_splitters(const U* img, uint32_t lo, uint32_t hi, U x, int upper) {
+  while (lo < hi) {
+    const uint32_t mid = (lo + hi) >> 1;
+    const U t = img[mid];
+    if (upper ? (t < x) : (t <= x))
+      lo = mid + 1;
+    else
+      hi = mid;
   }
-  const uint32_t b = i - (1u << depth);
-  return b < nspl ? b : nspl;
+  return lo;
 }
 
-// Loads nspl sorted splitters into every copy of the tree (image order),
-// padding with the largest image.  Every thread of the block takes part; a
+template <typename U>
+__device__ inline uint32_t tab_entry_of(U x, U base, uint32_t shift) {
+  if (x < base) return 0;
+  const U q = (U)(x - base) >> shift;
+  return q < (U)kSplitTab ? (uint32_t)q : kSplitTab - 1;
+}
+
+// Builds the table from nspl <= 255 sorted splitters (raw keys): one
+// workgroup of kSplitTabThreads threads.
+constexpr int kSplitTabThreads = 1024;
+
+template <typename U>
+__global__ __launch_bounds__(kSplitTabThreads) void split_tab_kernel(const U* __restrict__ spl, uint32_t nspl,
+                                                                     int upper, Xform<U> xf,
+                                                                     SplitTab<U>* __restrict__ T) {
+  __shared__ U s_img[kRadix];
+  __shared__ U s_base;
+  __shared__ uint32_t s_shift;
+  const uint32_t tid = threadIdx.x;
+  for (uint32_t j = tid; j < kRadix; j += kSplitTabThreads) {
+    const U im = j < nspl ? to_ordered(spl[j], xf) : U(~U(0));
+    s_img[j] = im;
+    T->img[j] = im;
+  }
+  __syncthreads();
+  if (tid == 0) {
+    const U base = nspl ? s_img[0] : U(0);
+    const U span = nspl ? (U)(s_img[nspl - 1] - base) : U(0);
+    uint32_t shift = 0;
+    while ((U)(span >> shift) >= (U)kSplitTab) ++shift;
+    s_base = base;
+    s_shift = shift;
+    T->base = base;
+    T->shift = shift;
+    T->nspl = nspl;
+  }
+  __syncthreads();
+  const U base = s_base;
+  const uint32_t shift = s_shift;
+  const U last = (U)(~U(0) - base) >> shift;  // entry of the largest image, before the clamp
+  for (uint32_t i = tid; i < kSplitTab; i += kSplitTabThreads) {
+    uint32_t lo, hi;
+    if (i != 0 && (U)i > last) {
+      lo = hi = nspl;  // no image maps here
+    } else {
+      const U first = i == 0 ? U(0) : (U)(base + ((U)i << shift));
+      const bool to_top = i == kSplitTab - 1 || (U)i == last;
+      const U end = to_top ? U(~U(0)) : (U)(base + ((U)(i + 1) << shift) - 1);
+      lo = count_splitters<U>(s_img, 0, nspl, first, upper);
+      hi = count_splitters<U>(s_img, lo, nspl, end, upper);
+    }
+    T->entry[i] = (uint16_t)(lo | (hi << 8));
+  }
+}
+
+// Copies the table's entries and splitter images into LDS (`ent`: kSplitTab
+// u16, `img`: kRadix images).  Every thread of the block takes part; a
 // barrier must follow.
 template <typename U>
-__device__ inline void load_tree(U* tree, const U* spl, uint32_t nspl, int depth, const Xform<U>& xf, uint32_t ncopy,
-                                 uint32_t tid, uint32_t nthreads) {
-  const uint32_t nodes = (1u << depth) - 1;
-  for (uint32_t x = tid; x < nodes * ncopy; x += nthreads) {
-    const uint32_t j = x / ncopy, c = x % ncopy;
-    tree[tree_node(j, depth) * ncopy + c] = j < nspl ? to_ordered(spl[j], xf) : U(~U(0));
+__device__ inline void load_split_tab(uint16_t* ent, U* img, const SplitTab<U>* T, uint32_t tid, uint32_t nthreads) {
+  const uint4* src = reinterpret_cast<const uint4*>(T->entry);
+  uint4* dst = reinterpret_cast<uint4*>(ent);
+  for (uint32_t i = tid; i < kSplitTab * 2 / 16; i += nthreads) dst[i] = src[i];
+  for (uint32_t j = tid; j < kRadix; j += nthreads) img[j] = T->img[j];
+}
+
+// Buckets of N images at once: every entry is read first, then the entries
+// that hold splitters are narrowed together, one branch-free step per key
+// per round (an idle key re-reads a valid image and keeps its range), so
+// the LDS reads of all N keys are in flight together.
+template <typename U, int N>
+__device__ inline void tab_buckets(const uint16_t* ent, const U* img, const U (&x)[N], U base, uint32_t shift,
+                                   int upper, uint32_t (&b)[N]) {
+  uint32_t lo[N], hi[N];
+#pragma unroll
+  for (int i = 0; i < N; ++i) {
+    const uint32_t e = ent[tab_entry_of<U>(x[i], base, shift)];
+    lo[i] = e & 0xFFu;
+    hi[i] = e >> 8;
   }
+  bool more = false;
+#pragma unroll
+  for (int i = 0; i < N; ++i) more |= lo[i] < hi[i];
+  while (more) {
+    more = false;
+#pragma unroll
+    for (int i = 0; i < N; ++i) {
+      const bool act = lo[i] < hi[i];
+      const uint32_t mid = (lo[i] + hi[i]) >> 1;  // < 256: a valid image even when idle
+      const U t = img[mid];
+      const bool right = upper ? (t < x[i]) : (t <= x[i]);
+      lo[i] = (act && right) ? mid + 1 : lo[i];
+      hi[i] = (act && !right) ? mid : hi[i];
+      more |= lo[i] < hi[i];
+    }
+  }
+#pragma unroll
+  for (int i = 0; i < N; ++i) b[i] = lo[i];
 }
 
 // Digit of a key: the ordered image's byte at `shift`.  IDENT is the
@@ -305,21 +390,21 @@ __global__ __launch_bounds__(THREADS) void onesweep_kernel(OnesweepArgs<U, V, S>
   constexpr size_t BUF_ELEM = (sizeof(U) > sizeof(V) || !KV) ? sizeof(U) : sizeof(V);
   // DUAL: keys and payloads are staged side by side (keys, then payloads),
   // and written out in one loop; the ranking tables then share the tile
-  // buffer.  Multisplit keeps one buffer reused for the payloads, and so do
-  // geometries where the larger buffer would leave fewer workgroups per CU
-  // than the 16 waves the registers allow (the 160 KB of LDS shared).
+  // buffer.  Geometries where the larger buffer would leave fewer workgroups
+  // per CU than the 16 waves the registers allow (the 160 KB of LDS shared)
+  // keep one buffer reused for the payloads.  Multisplit also keeps the
+  // bucket of every slot (a byte) for the write-out.
   constexpr bool DUAL =
-      KV && !SPLIT && (1024 / THREADS) * ((size_t)TILE * (sizeof(U) + sizeof(V)) + 2048) <= 160 * 1024;
+      KV && (1024 / THREADS) * ((size_t)TILE * (sizeof(U) + sizeof(V) + (SPLIT ? 1 : 0)) + 2048) <= 160 * 1024;
   constexpr size_t BUF_BYTES = DUAL ? (size_t)TILE * (sizeof(U) + sizeof(V)) : (size_t)TILE * BUF_ELEM;
 
   // s_wave_hist[w][d]: wave w's count of digit d during ranking, then the
   // tile-local slot of wave w's first key with digit d.
   __shared__ uint32_t s_wave_hist_own[DUAL ? 1 : WAVES * kRadix];
   // s_match[w][d]: lane mask of wave w's lanes holding digit d for the item
-  // being ranked; every entry is back to 0 after each item.  Passes other
-  // than multisplit keep it in the tile buffer, free until ranking ends,
-  // which leaves LDS for larger tiles.
-  constexpr bool MATCH_IN_BUF = !SPLIT;
+  // being ranked; every entry is back to 0 after each item.  It lives in the
+  // tile buffer, free until ranking ends, which leaves LDS for larger tiles.
+  constexpr bool MATCH_IN_BUF = true;
   __shared__ unsigned long long s_match_own[MATCH_IN_BUF ? 1 : WAVES * kRadix];
   __shared__ Idx s_gofs[kRadix];
   __shared__ uint32_t s_wsum[4];
@@ -332,12 +417,20 @@ __global__ __launch_bounds__(THREADS) void onesweep_kernel(OnesweepArgs<U, V, S>
       MATCH_IN_BUF ? reinterpret_cast<unsigned long long*>(s_buf) : s_match_own;
   uint32_t* const s_wave_hist =
       DUAL ? reinterpret_cast<uint32_t*>(s_buf + WAVES * kRadix * sizeof(unsigned long long)) : s_wave_hist_own;
-  // Multisplit: the splitter tree lives in the tile buffer until the buckets
-  // are found (before ranking; the buffer is written after it).  One copy:
-  // the 18 keys' descents of a thread overlap, and copying the tree per tile
-  // cost more than the deeper levels' bank conflicts.
-  U* const s_tree = reinterpret_cast<U*>(s_buf);
-  constexpr uint32_t tree_ncopy = 1;
+  // Multisplit: the splitters' bucket table (entries, then splitter images)
+  // sits in the tile buffer after the ranking tables until the buckets are
+  // found (before ranking; the buffer is written after it).
+  constexpr size_t TAB_OFF = WAVES * kRadix * (sizeof(unsigned long long) + (DUAL ? 4 : 0));
+  static_assert(!SPLIT || BUF_BYTES >= TAB_OFF + kSplitTab * 2 + kRadix * sizeof(U),
+                "bucket table fits the tile buffer beside the ranking tables");
+  uint16_t* const s_ent = reinterpret_cast<uint16_t*>(s_buf + TAB_OFF);
+  U* const s_img = reinterpret_cast<U*>(s_buf + TAB_OFF + kSplitTab * 2);
+  U tab_base = U(0);
+  uint32_t tab_shift = 0;
+  if constexpr (SPLIT) {
+    tab_base = a.split_tab->base;
+    tab_shift = a.split_tab->shift;
+  }
 
   uint32_t src;
   if constexpr (SPLIT) {
@@ -370,14 +463,10 @@ __global__ __launch_bounds__(THREADS) void onesweep_kernel(OnesweepArgs<U, V, S>
     s_wave_hist[i] = 0;
     s_match[i] = 0;
   }
-  if constexpr (SPLIT) load_tree<U>(s_tree, a.split_keys, a.nspl, a.depth, a.xf, tree_ncopy, tid, THREADS);
+  if constexpr (SPLIT) load_split_tab<U>(s_ent, s_img, a.split_tab, tid, THREADS);
   lds_barrier();
   // Digit of a key: its byte at `shift`, or its bucket in multisplit mode.
-  auto digit_of_key = [&](U key) -> uint32_t {
-    if constexpr (SPLIT)
-      return tree_bucket<U>(s_tree, to_ordered(key, a.xf), a.upper, a.nspl, a.depth, tree_ncopy, 0u);
-    return key_digit<U, IDENT>(key, a.xf, shift);
-  };
+  auto digit_of_key = [&](U key) -> uint32_t { return key_digit<U, IDENT>(key, a.xf, shift); };
   const uint32_t tile = s_tile;
   uint64_t tile_base, remain;
   uint32_t first = 0;       // first tile of this tile's segment
@@ -431,11 +520,16 @@ __global__ __launch_bounds__(THREADS) void onesweep_kernel(OnesweepArgs<U, V, S>
   // Buckets (multisplit) are searched once and kept packed, 4 per register.
   uint32_t bpk[SPLIT ? (ITEMS + 3) / 4 : 1];
   if constexpr (SPLIT) {
+    U img[ITEMS];
+    uint32_t bk[ITEMS];
+#pragma unroll
+    for (int i = 0; i < ITEMS; ++i) img[i] = to_ordered(k[i], a.xf);
+    tab_buckets<U, ITEMS>(s_ent, s_img, img, tab_base, tab_shift, a.upper, bk);
 #pragma unroll
     for (int q = 0; q < (ITEMS + 3) / 4; ++q) bpk[q] = 0;
 #pragma unroll
     for (int i = 0; i < ITEMS; ++i) {
-      const uint32_t d = (full || (lofs + i * kWave) < valid) ? digit_of_key(k[i]) : 255u;
+      const uint32_t d = (full || (lofs + i * kWave) < valid) ? bk[i] : 255u;
       bpk[i / 4] |= d << (8 * (i % 4));
     }
   }
@@ -549,6 +643,7 @@ __global__ __launch_bounds__(THREADS) void onesweep_kernel(OnesweepArgs<U, V, S>
     for (int i = 0; i < ITEMS; ++i) {
       s_k[r[i]] = k[i];
       s_vd[r[i]] = v[i];
+      if constexpr (SPLIT) s_bkt[r[i]] = (unsigned char)digit_at(i);
     }
   } else {
 #pragma unroll
@@ -608,7 +703,7 @@ __global__ __launch_bounds__(THREADS) void onesweep_kernel(OnesweepArgs<U, V, S>
       const uint32_t p = (uint32_t)j * THREADS + tid;
       if (p < valid) {
         const U key = s_k[p];
-        const Idx dst = s_gofs[digit_of_key(key)] + (Idx)p;
+        const Idx dst = s_gofs[SPLIT ? (uint32_t)s_bkt[p] : digit_of_key(key)] + (Idx)p;
         kout[dst] = key;
         vout[dst] = s_vd[p];
       }
@@ -627,7 +722,7 @@ __global__ __launch_bounds__(THREADS) void onesweep_kernel(OnesweepArgs<U, V, S>
     }
   }
 
-  // 7. payloads (multisplit) through the same slots.
+  // 7. payloads (geometries with one buffer) through the same slots.
   if constexpr (KV) {
     lds_barrier();
     V* s_v = reinterpret_cast<V*>(s_buf);
@@ -742,37 +837,42 @@ static inline uint32_t tile_keys_for(int kb, int vb) {
 }
 
 
-// Multisplit passes: as Geometry, except keys-only 32-bit keys at 1024 x 18,
-// 64-bit keys with no or 32-bit payloads at 1024 x 10 and 32-bit keys with
-// 64-bit payloads at 512 x 12 (the bucket byte per slot, its own match table
-// and one buffer reused for the payloads do not fit beside the larger tiles).
+// Multisplit passes keep a bucket byte per slot beside the side-by-side
+// tile: 32-bit keys alone at 1024 x 22 (registers), with 32-bit payloads at
+// 1024 x 17, 64-bit keys alone at 1024 x 14, and 12-byte key + payload at
+// 1024 x 11 (64-bit keys with 64-bit payloads as Geometry, 1024 x 9).
 template <typename U, int VB>
 struct SplitGeometry {
   static constexpr int THREADS = Geometry<U, VB>::THREADS, ITEMS = Geometry<U, VB>::ITEMS;
 };
 template <>
 struct SplitGeometry<uint32_t, 0> {
-  static constexpr int THREADS = 1024, ITEMS = 18;
+  static constexpr int THREADS = 1024, ITEMS = 22;
+};
+template <>
+struct SplitGeometry<uint32_t, 4> {
+  static constexpr int THREADS = 1024, ITEMS = 17;
 };
 template <>
 struct SplitGeometry<uint32_t, 8> {
-  static constexpr int THREADS = 512, ITEMS = 12;
+  static constexpr int THREADS = 1024, ITEMS = 11;
 };
 template <>
 struct SplitGeometry<uint64_t, 0> {
-  static constexpr int THREADS = 1024, ITEMS = 10;
+  static constexpr int THREADS = 1024, ITEMS = 14;
 };
 template <>
 struct SplitGeometry<uint64_t, 4> {
-  static constexpr int THREADS = 1024, ITEMS = 10;
+  static constexpr int THREADS = 1024, ITEMS = 11;
 };
 
 static inline uint32_t split_tile_keys(int kb, int vb) {
   if (kb == 4 && vb == 0) return SplitGeometry<uint32_t, 0>::THREADS * SplitGeometry<uint32_t, 0>::ITEMS;
+  if (kb == 4 && vb == 4) return SplitGeometry<uint32_t, 4>::THREADS * SplitGeometry<uint32_t, 4>::ITEMS;
+  if (kb == 4 && vb == 8) return SplitGeometry<uint32_t, 8>::THREADS * SplitGeometry<uint32_t, 8>::ITEMS;
   if (kb == 8 && vb == 0) return SplitGeometry<uint64_t, 0>::THREADS * SplitGeometry<uint64_t, 0>::ITEMS;
   if (kb == 8 && vb == 4) return SplitGeometry<uint64_t, 4>::THREADS * SplitGeometry<uint64_t, 4>::ITEMS;
-  if (kb == 4 && vb == 8) return SplitGeometry<uint32_t, 8>::THREADS * SplitGeometry<uint32_t, 8>::ITEMS;
-  return tile_keys_for(kb, vb);
+  return SplitGeometry<uint64_t, 8>::THREADS * SplitGeometry<uint64_t, 8>::ITEMS;
 }
 
 // Tile geometries of the segmented sort: 0 = the single-array geometry above
@@ -1199,30 +1299,36 @@ hipError_t radix_sort_segments(int kb, int vb, void* keys, void* vals, const Seg
 // perfect splitter tree in LDS.  Output goes to separate buffers, bucket after bucket,
 // keys of a bucket in input order.
 
-// Bucket counts of a multisplit in one read of the keys: kTreeCopies copies
-// of the splitter tree and of the 256 counters, each lane using the copy of
-// its bank (lane % 32), so neither the tree descent nor the count adds
-// conflict across lanes.
+// Bucket counts of a multisplit in one read of the keys: buckets from the
+// bucket table in LDS, counted in kCountCopies copies of the 256 counters,
+// lane l using copy l % 32 (its bank), so the counts add no conflict across
+// lanes.
 constexpr int kSplitHistThreads = 1024;
+constexpr uint32_t kCountCopies = 32;
 
 template <typename U>
 __global__ __launch_bounds__(kSplitHistThreads) void split_hist_kernel(
-    const U* __restrict__ keys, uint64_t n, Xform<U> xf, const U* __restrict__ spl, uint32_t nspl, int upper,
-    int depth, unsigned long long* __restrict__ g_hist, uint4* __restrict__ zero_ptr, uint64_t zero_vec4) {
-  __shared__ __attribute__((aligned(16))) uint32_t s_cnt[kRadix * kTreeCopies];  // [bucket][copy]
-  __shared__ U s_tree[kRadix * kTreeCopies];
+    const U* __restrict__ keys, uint64_t n, Xform<U> xf, const SplitTab<U>* __restrict__ T, int upper,
+    unsigned long long* __restrict__ g_hist, uint4* __restrict__ zero_ptr, uint64_t zero_vec4) {
+  __shared__ __attribute__((aligned(16))) uint32_t s_cnt[kRadix * kCountCopies];  // [bucket][copy]
+  __shared__ __attribute__((aligned(16))) uint16_t s_ent[kSplitTab];
+  __shared__ U s_img[kRadix];
   const uint32_t tid = threadIdx.x;
-  const uint32_t copy = tid & (kTreeCopies - 1);
+  const uint32_t copy = tid & (kCountCopies - 1);
   uint4* c4 = reinterpret_cast<uint4*>(s_cnt);
-  for (uint32_t i = tid; i < kRadix * kTreeCopies / 4; i += kSplitHistThreads) c4[i] = make_uint4(0, 0, 0, 0);
-  load_tree<U>(s_tree, spl, nspl, depth, xf, kTreeCopies, tid, kSplitHistThreads);
+  for (uint32_t i = tid; i < kRadix * kCountCopies / 4; i += kSplitHistThreads) c4[i] = make_uint4(0, 0, 0, 0);
+  load_split_tab<U>(s_ent, s_img, T, tid, kSplitHistThreads);
+  const U tab_base = T->base;
+  const uint32_t tab_shift = T->shift;
   const uint64_t gtid = (uint64_t)blockIdx.x * kSplitHistThreads + tid;
   const uint64_t gstride = (uint64_t)gridDim.x * kSplitHistThreads;
   for (uint64_t i = gtid; i < zero_vec4; i += gstride) zero_ptr[i] = make_uint4(0, 0, 0, 0);
   __syncthreads();
   auto count = [&](U key) {
-    const uint32_t b = tree_bucket<U>(s_tree, to_ordered(key, xf), upper, nspl, depth, kTreeCopies, copy);
-    atomicAdd(&s_cnt[b * kTreeCopies + copy], 1u);
+    U x[1] = {to_ordered(key, xf)};
+    uint32_t b[1];
+    tab_buckets<U, 1>(s_ent, s_img, x, tab_base, tab_shift, upper, b);
+    atomicAdd(&s_cnt[b[0] * kCountCopies + copy], 1u);
   };
   // 16-byte loads, two in flight per thread.
   using VT = typename Vec16<U>::T;
@@ -1248,13 +1354,13 @@ __global__ __launch_bounds__(kSplitHistThreads) void split_hist_kernel(
   if (tid < kRadix) {
     uint32_t sum = 0;
 #pragma unroll 8
-    for (uint32_t c = 0; c < kTreeCopies; ++c) sum += s_cnt[tid * kTreeCopies + ((c + tid) & (kTreeCopies - 1))];
+    for (uint32_t c = 0; c < kCountCopies; ++c) sum += s_cnt[tid * kCountCopies + ((c + tid) & (kCountCopies - 1))];
     if (sum) atomicAdd(&g_hist[tid], (unsigned long long)sum);
   }
 }
 
 struct SplitLayout {
-  size_t off_ctrl, off_hist, off_base, off_status, total;
+  size_t off_ctrl, off_hist, off_base, off_tab, off_status, total;
   uint64_t tiles;
   size_t status_word;
 };
@@ -1270,6 +1376,8 @@ static SplitLayout split_layout(int kb, int vb, uint64_t n) {
   o = align_up(o + kRadix * sizeof(unsigned long long), 256);
   L.off_base = o;
   o = align_up(o + kRadix * sizeof(uint64_t), 256);
+  L.off_tab = o;
+  o = align_up(o + sizeof(SplitTab<uint64_t>), 256);  // the larger of the two key widths
   L.off_status = o;
   o = align_up(o + L.tiles * kRadix * L.status_word, 256);
   L.total = o;
@@ -1301,11 +1409,11 @@ static hipError_t multisplit_t(const U* keys, const void* vals, uint64_t n, cons
   uint64_t blocks = (n + kSplitHistThreads * 16 - 1) / (kSplitHistThreads * 16);
   if (blocks < 1) blocks = 1;
   if (blocks > 512) blocks = 512;
+  SplitTab<U>* tab = reinterpret_cast<SplitTab<U>*>(ws + L.off_tab);
+  hipLaunchKernelGGL(split_tab_kernel<U>, dim3(1), dim3(kSplitTabThreads), 0, stream, d_spl, nspl, upper, xf, tab);
   if (timer) timer->begin("split_hist", n * sizeof(U) + status_bytes, stream);
-  const int depth = tree_depth(nspl);
   hipLaunchKernelGGL(split_hist_kernel<U>, dim3((uint32_t)blocks), dim3(kSplitHistThreads), 0, stream, keys, n, xf,
-                     d_spl, nspl,
-                     upper, depth, hist, reinterpret_cast<uint4*>(status), status_bytes / 16);
+                     tab, upper, hist, reinterpret_cast<uint4*>(status), status_bytes / 16);
   if (timer) timer->end(stream);
   hipLaunchKernelGGL(radix_scan_kernel, dim3(1), dim3(256), 0, stream, hist, base, ctrl, 1, n);
   OnesweepArgs<U, V, S> a{};
@@ -1321,10 +1429,8 @@ static hipError_t multisplit_t(const U* keys, const void* vals, uint64_t n, cons
   a.status = status;
   a.ctrl = ctrl;
   a.tiles = (uint32_t)L.tiles;
-  a.split_keys = d_spl;
-  a.nspl = nspl;
+  a.split_tab = tab;
   a.upper = upper;
-  a.depth = depth;
   a.group = tile_group(stream, policy);
   if (timer) timer->begin("multisplit_pass", 2 * n * (sizeof(U) + (KV ? sizeof(V) : 0)), stream);
   hipLaunchKernelGGL((onesweep_kernel<U, V, KV, THREADS, ITEMS, S, false, false, Idx, true>), dim3((uint32_t)L.tiles),

@@ -1,5 +1,5 @@
-"""Multisplit (gs_multisplit): stable bucket partition of unsorted keys by an
-LDS splitter tree, against the oracle's restatement (std::lower_bound /
+"""Multisplit (gs_multisplit): stable bucket partition of unsorted keys by the
+splitters' bucket table, against the oracle's restatement (std::lower_bound /
 std::upper_bound bucket + stable counting partition), and its offsets against
 gs_bucket_bounds of the sorted keys."""
 from __future__ import annotations
@@ -37,10 +37,15 @@ def splitters_from(keys, nspl, seed, descending):
 
 def run(handle, kdt, vdt, n, nspl, upper, descending, seed, distinct=None):
     keys = keys_of(kdt, n, seed, distinct)
-    vals = None if vdt is None else np.arange(n, dtype=np.uint64).astype(vdt)
     spl = splitters_from(keys, nspl, seed + 7, descending)
     say(f"multisplit kdt={np.dtype(kdt).name} vdt={vdt and np.dtype(vdt).name} n={n} nspl={nspl} "
         f"upper={upper} desc={descending} seed={seed}")
+    check(handle, keys, vdt, spl, upper, descending)
+
+
+def check(handle, keys, vdt, spl, upper, descending):
+    n = keys.size
+    vals = None if vdt is None else np.arange(n, dtype=np.uint64).astype(vdt)
     tk = dev(keys)
     ok = torch.empty_like(tk)
     tv = None if vals is None else dev(vals)
@@ -92,3 +97,39 @@ def test_offsets_equal_bounds_of_sorted(handle):
         assert_bits_equal(host(ok), keys, "sorted input stays in place")
         b = handle.bucket_bounds(tk, np.array([0, n], dtype=np.uint64), spl, upper=upper)
         assert np.array_equal(np.asarray(b).reshape(-1).astype(np.int64), offs.astype(np.int64))
+
+
+def edge_values(kdt):
+    d = np.dtype(kdt)
+    if d.kind == "f":
+        return np.array([-np.finfo(d).max, np.finfo(d).max], dtype=d)
+    info = np.iinfo(d)
+    return np.array([info.min, info.max], dtype=d)
+
+
+@pytest.mark.parametrize("kdt", [np.uint32, np.int32, np.uint64, np.int64, np.float32, np.float64],
+                         ids=lambda d: np.dtype(d).name)
+@pytest.mark.parametrize("upper", [False, True], ids=["lower_cut", "upper_cut"])
+def test_bucket_table_edges(handle, kdt, upper):
+    """Splitter sets that stress the bucket table: packed into a narrow range
+    of a wide key spread (most table entries hold many splitters), at the
+    type's extreme values, all equal, and far from most keys."""
+    n = 400_003
+    keys = keys_of(kdt, n, 31)
+    lo, hi = edge_values(kdt)
+    mid = np.sort(keys)[n // 2]
+    near = np.sort(keys)[n // 2 - 40: n // 2 + 40]  # 80 consecutive distinct-ish keys
+    cases = {
+        "narrow": np.sort(near[::2]),
+        "extremes": np.sort(np.array([lo, lo, mid, hi, hi], dtype=keys.dtype)),
+        "all_equal": np.full(17, mid, dtype=keys.dtype),
+        "far_low": np.sort(np.concatenate([np.full(3, lo, dtype=keys.dtype), near[:5]])),
+        "full_255": np.sort(keys[(datagen.draws(77, 255) % np.uint64(n)).astype(np.int64)]),
+    }
+    # keys equal to the extremes and to the splitters, on purpose
+    keys = keys.copy()
+    keys[:4] = [lo, hi, mid, near[3]]
+    for name, spl in cases.items():
+        say(f"bucket table edge {name} kdt={np.dtype(kdt).name} upper={upper} nspl={spl.size}")
+        check(handle, keys, np.uint32, spl, upper, descending=False)
+        check(handle, keys, None, spl[::-1].copy(), upper, descending=True)
