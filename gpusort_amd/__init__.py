@@ -19,6 +19,7 @@ LIB_PATH = _HERE / "libgpusort_hip.so"
 # gs_dtype codes (include/gpusort_hip.h)
 I32, U32, F32, I64, U64, F64, NONE = 0, 1, 2, 3, 4, 5, 255
 HYKSORT, SAMPLESORT, HYPERQUICKSORT = 0, 1, 2
+TILES_AUTO, TILES_ARRIVAL, TILES_GROUPED = -1, 0, 1  # gs_set_tile_order
 # GS_NOALLOC_SCRATCH_BYTES: workspace bytes of gs_sort_pairs_noalloc
 NOALLOC_SCRATCH_BYTES = 72 << 20
 
@@ -173,6 +174,7 @@ _SIGS = {
     "gs_create": ([ctypes.POINTER(ctypes.c_void_p), ctypes.c_int], ctypes.c_int),
     "gs_destroy": ([ctypes.c_void_p], ctypes.c_int),
     "gs_set_stream": ([ctypes.c_void_p, ctypes.c_void_p], ctypes.c_int),
+    "gs_set_tile_order": ([ctypes.c_void_p, ctypes.c_int], ctypes.c_int),
     "gs_synchronize": ([ctypes.c_void_p], ctypes.c_int),
     "gs_arena_bytes": ([ctypes.c_void_p, ctypes.POINTER(ctypes.c_uint64)], ctypes.c_int),
     "gs_release_arena": ([ctypes.c_void_p], ctypes.c_int),
@@ -339,6 +341,11 @@ class Handle:
     def set_stream(self, stream) -> None:
         ptr = stream if isinstance(stream, int) else getattr(stream, "cuda_stream", 0)
         _check(self.lib.gs_set_stream(self.h, ctypes.c_void_p(ptr)))
+
+    def set_tile_order(self, order: int) -> None:
+        """TILES_AUTO (grouped tiles on the null stream, arrival order
+        elsewhere), TILES_ARRIVAL or TILES_GROUPED (gs_set_tile_order)."""
+        _check(self.lib.gs_set_tile_order(self.h, int(order)))
 
     def use_torch_stream(self) -> None:
         import torch

@@ -225,6 +225,17 @@ gs_status gs_set_stream(gs_handle* h, void* stream) {
   return GS_OK;
 }
 
+gs_status gs_set_tile_order(gs_handle* h, int order) {
+  GS_CHECK_HANDLE(h);
+  if (order != GS_TILES_AUTO && order != GS_TILES_ARRIVAL && order != GS_TILES_GROUPED)
+    return fail(GS_INVALID_ARG, "gs_set_tile_order: unknown order %d", order);
+  static_assert(GS_TILES_AUTO == kTilesByStream && GS_TILES_ARRIVAL == kTilesArrival &&
+                    GS_TILES_GROUPED == kTilesGrouped,
+                "the C ABI's tile orders are the library's tile policies");
+  h->tile_policy = order;
+  return GS_OK;
+}
+
 gs_status gs_synchronize(gs_handle* h) {
   GS_CHECK_HANDLE(h);
   DeviceScope ds(h->device);

@@ -73,6 +73,21 @@ gs_status gs_get_device_count(int* count);
 gs_status gs_create(gs_handle** out, int device);
 gs_status gs_destroy(gs_handle* h);
 gs_status gs_set_stream(gs_handle* h, void* hip_stream);
+
+/* Tile hand-out of the handle's radix passes (sorts, segmented sorts,
+ * multisplit).  A pass's tiles finish in tile order, so a tile may only wait
+ * on tiles that started workgroups hold.  GS_TILES_AUTO (the default): tiles
+ * in XCD-grouped runs of 16 on the null stream, which assumes the sort has
+ * most of the GPU's workgroup slots (about 112 resident workgroups), and
+ * one arrival-order ticket on any other stream, which makes progress
+ * whatever else runs.  GS_TILES_ARRIVAL: the arrival-order ticket always,
+ * for callers that run other work beside null-stream sorts (non-blocking
+ * streams, other processes).  GS_TILES_GROUPED: grouped tiles on any
+ * stream, for callers that guarantee the sort has the GPU to itself.
+ * No counterpart in the reference (its seam is thrust's sort, which makes
+ * no such choice visible). */
+enum { GS_TILES_AUTO = -1, GS_TILES_ARRIVAL = 0, GS_TILES_GROUPED = 1 };
+gs_status gs_set_tile_order(gs_handle* h, int order);
 gs_status gs_synchronize(gs_handle* h);
 /* Bytes currently held by the handle's arena. */
 gs_status gs_arena_bytes(gs_handle* h, uint64_t* bytes);

@@ -198,3 +198,53 @@ def test_repeat_calls_reuse_arena(handle):
         handle.sort_keys(t)
         assert_bits_equal(host(t), np.sort(k2), f"repeat {s}")
     assert handle.arena_bytes() == before
+
+
+def test_tile_order_choices(handle):
+    """gs_set_tile_order: arrival-order tiles on the null stream and grouped
+    tiles on a side stream give the same results as the default choice;
+    an unknown order is refused."""
+    import torch
+
+    import gpusort_amd as g
+
+    n = (1 << 22) + 19457
+    keys = make_keys(np.uint32, n, seed=77)
+    vals = np.arange(n, dtype=np.uint32)
+    wk, wv = oracle.stable_sort_pairs(keys, vals)
+    side = torch.cuda.Stream()
+    try:
+        say(f"tile order arrival, null stream, pairs n={n}")
+        handle.set_tile_order(g.TILES_ARRIVAL)
+        tk, tv = dev(keys), dev(vals)
+        handle.sort_pairs(tk, tv)
+        assert_bits_equal(host(tk), wk, "arrival keys")
+        assert_bits_equal(host(tv), wv, "arrival values")
+        say("tile order arrival, segments and multisplit")
+        off = np.array([0, 5000, 5000, n // 2, n], dtype=np.uint64)
+        tk, tv = dev(keys), dev(vals)
+        handle.sort_segments(tk, torch.from_numpy(off.view(np.int64)).cuda(), tv)
+        gk, gv = host(tk), host(tv)
+        for a, b in zip(off[:-1].astype(np.int64), off[1:].astype(np.int64)):
+            sk, sv = oracle.stable_sort_pairs(keys[a:b], vals[a:b])
+            assert_bits_equal(gk[a:b], sk, f"segment keys [{a},{b})")
+            assert_bits_equal(gv[a:b], sv, f"segment values [{a},{b})")
+        spl = np.sort(keys[:: n // 33])[1:33]
+        ok, ov = torch.empty_like(dev(keys)), torch.empty_like(dev(vals))
+        offs = handle.multisplit(dev(keys), spl, ok, dev(vals), ov)
+        idx, want_offs = oracle.multisplit(keys, spl, False, False)
+        assert np.array_equal(offs.astype(np.int64), want_offs)
+        assert_bits_equal(host(ok), keys[idx], "multisplit keys")
+        say(f"tile order grouped, side stream, keys n={n}")
+        handle.set_tile_order(g.TILES_GROUPED)
+        handle.set_stream(side.cuda_stream)
+        with torch.cuda.stream(side):
+            t = dev(keys)
+            handle.sort_keys(t)
+            side.synchronize()
+        assert_bits_equal(host(t), oracle.merge_sort(keys), "grouped keys on a side stream")
+        with pytest.raises(Exception):
+            handle.set_tile_order(7)
+    finally:
+        handle.set_stream(0)
+        handle.set_tile_order(g.TILES_AUTO)
