@@ -86,6 +86,8 @@ CASES = [
     ("hyksort", 32, 16, 8, np.uint64, True, False, None),  # 16 runs per destination: two merge passes
     ("hyksort", 64, 64, 4, np.int32, False, True, None),   # one round of 64 buckets: two merge passes
     ("hyksort", 16, 2, 1, np.uint32, True, False, None),   # one sample per rank: refined over iterations
+    ("hyksort", 8, 4, 16, np.uint32, True, False, 1),      # all keys equal: selection ends on empty draws
+    ("hyksort", 16, 8, 8, np.uint64, False, True, 1),
     ("samplesort", 8, 0, 0, np.uint32, True, False, None),
     ("samplesort", 64, 0, 0, np.uint64, True, False, None),
     ("samplesort", 12, 0, 0, np.int64, False, True, None),
