@@ -441,6 +441,40 @@ def run_global_index(h, g, torch, n: int, steps: int = 5):
     return out
 
 
+def run_host_pointer(h, g, torch, n: int, reps: int = 2):
+    """The host-memory form (gs_sort_pairs_host, CudaUtils::SortByKey's
+    seam): copy in, sort, copy out, synchronised, on 2^28 u32 keys + u32
+    payloads in pinned and in pageable host memory.  PCIe-inclusive: this
+    is never the headline value."""
+    keys_d = torch.empty(n, dtype=torch.uint32, device="cuda")
+    h.generate(keys_d, g.GEN_UNIFORM_U32, SEED)
+    src_k = keys_d.cpu()
+    del keys_d
+    out = {"n": n}
+    for kind in ("pinned", "pageable"):
+        pin = kind == "pinned"
+        hk = torch.empty(n, dtype=torch.uint32, pin_memory=pin)
+        hv = torch.empty(n, dtype=torch.uint32, pin_memory=pin)
+        times = []
+        for _ in range(reps if pin else 1):
+            hk.copy_(src_k)
+            hv.copy_(torch.arange(n, dtype=torch.int64).to(torch.uint32))
+            t0 = time.perf_counter()
+            h.sort_pairs_host(hk.numpy(), hv.numpy())
+            times.append(time.perf_counter() - t0)
+        k_np, v_np = hk.numpy(), hv.numpy()
+        ok = bool(np.all(k_np[1:] >= k_np[:-1]))
+        idx = v_np.astype(np.int64)
+        ok = ok and bool(np.array_equal(src_k.numpy()[idx], k_np))
+        ok = ok and bool(np.all((idx[1:] > idx[:-1]) | (k_np[1:] != k_np[:-1])))
+        t = float(np.median(times))
+        out[kind] = {"ms": round(t * 1e3, 2), "gkeys_s": round(n / t / 1e9, 3),
+                     "GB_s_each_way": round(n * 8 / (t / 2) / 1e9, 1), "verified": ok}
+        del hk, hv
+    torch.cuda.empty_cache()
+    return out
+
+
 def run_c1(h, g, torch, with_cpu: bool):
     """Plain single-rank sort: 2^24 uniform u32 keys from seed 1, keys only;
     the GPU result must equal the reference CPU output (OmpUtils::MergeSort,
@@ -768,7 +802,7 @@ def main() -> int:
     ap.add_argument("--cpu-n", type=int, default=1 << 28, help="CPU baseline sample size")
     ap.add_argument("--copy-calib", action="store_true",
                     help="only run plain copies of --n uint32 (counter calibration) and exit")
-    ap.add_argument("--only", choices=["vrank", "batched", "multisplit", "global-index", "keys", "kv"],
+    ap.add_argument("--only", choices=["vrank", "batched", "multisplit", "global-index", "keys", "kv", "host"],
                     help="run one of the other configurations alone (for profiling) and print its JSON")
     args = ap.parse_args()
 
@@ -807,6 +841,8 @@ def main() -> int:
             r = run_batched(h, g, torch, args.arrays, 3)
         elif args.only == "multisplit":
             r = run_multisplit(h, g, torch, n)
+        elif args.only == "host":
+            r = run_host_pointer(h, g, torch, n)
         else:
             r = run_global_index(h, g, torch, n // 2)
         print(json.dumps({"only": args.only, "result": r}), flush=True)
@@ -857,6 +893,8 @@ def main() -> int:
         extra["served_mixed"] = run_served(h, g, torch, args.requests)
         log("[bench] served from host memory: 100 requests, serial vs served")
         extra["served_host_memory"] = run_served_host(h, g, torch, 100)
+        log("[bench] host-pointer KV sort 2^28 u32+u32 (copy in, sort, copy out)")
+        extra["host_pointer_kv_u32_2^28"] = run_host_pointer(h, g, torch, n)
         log("[bench] KV f32 normal 2^27")
         r = run_workload(h, g, torch, "normal_f32", n // 2, 3, 1, payload=True, profile=False)
         extra["kv_f32_normal_2^27"] = {"gkeys_s": round(r["gkeys_s"], 3), "ms": round(r["ms_per_sort"], 3),
