@@ -932,6 +932,9 @@ def main() -> int:
             "avg_ms": round(hist["total_ms"] / hist["launches"], 4),
             "GB/s": round(hist["bytes"] / hist["launches"] / (hist["total_ms"] / hist["launches"] * 1e-3) / 1e9, 1),
         },
+        # keys / (sum of the sort's kernel times per sort: histogram + passes), beside the wall-clock value
+        "gkeys_s_kernel_sum": None if not hist or roof is None else round(
+            n / ((hist["total_ms"] + roof["avg_launch_ms"] * roof["launches"]) / args.steps * 1e-3) / 1e9, 3),
         "cpu_baseline": cpu,
         "extra": extra,
     }
