@@ -230,6 +230,27 @@ void CaseHykSortKeyFunc() {
   Report("hyksort_keyfunc_user_type_one_rank", ok);
 }
 
+// A user element type over several ranks: the key function's values are
+// sorted through the virtual ranks' rounds with their global index, so equal
+// keys keep input order across ranks as well.
+void CaseHykSortKeyFuncRanks(const char* name, size_t p, unsigned k_way) {
+  const size_t n = 40000;
+  std::vector<Record> list(n);
+  for (size_t i = 0; i < n; ++i) {
+    list[i].weight = static_cast<double>(Draw(21, i) % 1511) - 700.0;
+    list[i].id = static_cast<int32_t>(i);
+    std::snprintf(list[i].tag, sizeof(list[i].tag), "e%zu", i);
+  }
+  std::vector<Record> want = list;
+  std::stable_sort(want.begin(), want.end(), [](const Record& a, const Record& b) { return a.weight < b.weight; });
+  MpiComm comm = MpiComm::Virtual(Sizes(p, n, 22));
+  gpusort::HykSort<double>(comm, k_way, true, [](const Record& r) { return r.weight; }, &list);
+  bool ok = Partitioned(comm, n) && comm.LastRounds() >= 1;
+  for (size_t i = 0; ok && i < n; ++i)
+    ok = list[i].id == want[i].id && list[i].weight == want[i].weight && strcmp(list[i].tag, want[i].tag) == 0;
+  Report(name, ok);
+}
+
 void CaseNonPow2FallsBackToSampleSort() {
   const size_t p = 6, n = 60000;
   std::vector<int64_t> k = Keys<int64_t>(15, n);
@@ -399,6 +420,8 @@ int main(int argc, char** argv) {
   CaseOneRankPairOrder("hyksort_one_rank_cpu_lexicographic_desc", std::greater<int32_t>(), false);
   CaseHykSortVariadic();
   CaseHykSortKeyFunc();
+  CaseHykSortKeyFuncRanks("hyksort_keyfunc_user_type_p8_k2", 8, 2);
+  CaseHykSortKeyFuncRanks("hyksort_keyfunc_user_type_p16_k4", 16, 4);
   CaseNonPow2FallsBackToSampleSort();
   CaseSampleSortPairs();
   CaseHyperQuickSort();

@@ -33,4 +33,4 @@ def test_dropin_cases_on_device():
     print(r.stdout)
     assert r.returncode == 0, r.stdout + r.stderr
     assert "FAIL" not in r.stdout
-    assert r.stdout.count("PASS") == 24
+    assert r.stdout.count("PASS") == 26
