@@ -238,7 +238,7 @@ def run_batched(h, g, torch, n_arrays: int, steps: int, seed: int = 4):
         "graph_replay_ms": round(float(np.median(graph_ms)), 3),
         "graph_replay_gkeys_s": round(n / (float(np.median(graph_ms)) * 1e-3) / 1e9, 3),
         "tuned_size_classes": tuned,
-        "tuning_table_ms": [(r["small_max"], r["medium_max"], round(r["ms"], 3)) for r in table],
+        "tuning_table_ms": [(r["small_max"], r["medium_max"], r["medium_geometry"], round(r["ms"], 3)) for r in table],
         "verified": same and spot and graph_same,
         "classes": classes,
         "seg_onesweep_pass": pass_stats,

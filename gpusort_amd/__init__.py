@@ -123,7 +123,7 @@ class ServerStats(ctypes.Structure):
 
 
 class SegConfig(ctypes.Structure):
-    _fields_ = [("small_max", ctypes.c_uint32), ("medium_max", ctypes.c_uint64)]
+    _fields_ = [("small_max", ctypes.c_uint32), ("medium_max", ctypes.c_uint64), ("medium_geometry", ctypes.c_uint32)]
 
 
 class TuneRow(ctypes.Structure):
@@ -401,10 +401,10 @@ class Handle:
     def seg_config(self, kdt: int, vdt: int = NONE) -> dict:
         c = SegConfig()
         _check(self.lib.gs_get_seg_config(self.h, kdt, vdt, ctypes.byref(c)))
-        return {"small_max": c.small_max, "medium_max": c.medium_max}
+        return {"small_max": c.small_max, "medium_max": c.medium_max, "medium_geometry": c.medium_geometry}
 
-    def set_seg_config(self, kdt: int, vdt: int, small_max: int, medium_max: int) -> None:
-        c = SegConfig(small_max, medium_max)
+    def set_seg_config(self, kdt: int, vdt: int, small_max: int, medium_max: int, medium_geometry: int = 0) -> None:
+        c = SegConfig(small_max, medium_max, medium_geometry)
         _check(self.lib.gs_set_seg_config(self.h, kdt, vdt, ctypes.byref(c)))
 
     def autotune_segments(self, kdt: int, vdt: int = NONE, total_keys: int = 1 << 25, median: float = 2048.0,
@@ -416,8 +416,9 @@ class Handle:
         _check(self.lib.gs_autotune_segments(self.h, kdt, vdt, total_keys, median, sigma, ctypes.byref(best), rows,
                                              64, ctypes.byref(nr)))
         table = [{"small_max": rows[i].config.small_max, "medium_max": rows[i].config.medium_max,
-                  "ms": rows[i].ms} for i in range(nr.value)]
-        return {"small_max": best.small_max, "medium_max": best.medium_max}, table
+                  "medium_geometry": rows[i].config.medium_geometry, "ms": rows[i].ms} for i in range(nr.value)]
+        return {"small_max": best.small_max, "medium_max": best.medium_max,
+                "medium_geometry": best.medium_geometry}, table
 
     def seg_plan(self, offsets: np.ndarray, kdt: int, vdt: int = NONE, descending: bool = False) -> "SegPlan":
         return SegPlan(self, offsets, kdt, vdt, descending)

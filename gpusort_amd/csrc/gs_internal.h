@@ -78,7 +78,7 @@ hipError_t radix_sort(int key_bytes, int val_bytes, void* keys, void* vals, uint
 // ping-pong buffer in the workspace mirrors the whole array); n_keys is the
 // number of keys in the segments.
 struct SegTiles {
-  int geo;  // tile geometry (radix_seg_tile_keys): 0 = the large one, 1 = the small one
+  int geo;  // tile geometry (radix_seg_tile_keys): 0 = the large one, 1-3 = the small-tile class's
   const uint32_t* tile_seg;
   const uint32_t* seg_first;
   const uint64_t* seg_range;
@@ -109,6 +109,7 @@ uint32_t radix_seg_tile_keys(int key_bytes, int val_bytes, int geo);
 struct SegConfig {
   uint32_t small_max;
   uint64_t medium_max;
+  uint32_t medium_geometry = 0;  // small-tile class: 0 default, 1 half keys per thread, 2 twice the threads
 };
 SegConfig default_seg_config(int key_bytes, int val_bytes);
 

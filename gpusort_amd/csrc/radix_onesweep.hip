@@ -892,13 +892,27 @@ template <typename U, int VB>
 struct SegGeometry<U, VB, 1> {
   static constexpr int THREADS = 256, ITEMS = (sizeof(U) == 4 && VB <= 4) ? 16 : 12;
 };
+// The medium class's other choices for the autotuner: half the keys per
+// thread (2: 256 threads x 8 / x 6), or twice the threads (3: 512 x 16 /
+// x 12).
+template <typename U, int VB>
+struct SegGeometry<U, VB, 2> {
+  static constexpr int THREADS = 256, ITEMS = SegGeometry<U, VB, 1>::ITEMS / 2;
+};
+template <typename U, int VB>
+struct SegGeometry<U, VB, 3> {
+  static constexpr int THREADS = 512, ITEMS = SegGeometry<U, VB, 1>::ITEMS;
+};
 
 uint32_t radix_seg_tile_keys(int kb, int vb, int geo) {
   if (geo == 0) {
     if (kb == 4 && vb == 0) return SegGeometry<uint32_t, 0, 0>::THREADS * SegGeometry<uint32_t, 0, 0>::ITEMS;
     return tile_keys_for(kb, vb);
   }
-  return (kb == 4 && vb <= 4) ? 256 * 16 : 256 * 12;
+  const uint32_t items = (kb == 4 && vb <= 4) ? 16 : 12;
+  if (geo == 2) return 256 * (items / 2);
+  if (geo == 3) return 512 * items;
+  return 256 * items;
 }
 
 struct RadixLayout {
@@ -1204,6 +1218,9 @@ static hipError_t radix_segments_t(U* keys, void* vals_v, const SegTiles& T, con
   uint32_t* cnt = reinterpret_cast<uint32_t*>(ws + L.off_cnt);
   uint32_t* status = reinterpret_cast<uint32_t*>(ws + L.off_status);
   const uint64_t status_per_pass = T.tiles * kRadix;
+  // The plan's tiles were cut with radix_seg_tile_keys: it must be this
+  // geometry's tile, or the tile maps and status words would not fit it.
+  if (radix_seg_tile_keys((int)sizeof(U), VB, G) != (uint32_t)(THREADS * ITEMS)) return hipErrorInvalidValue;
 
   if (T.n_multi) {
     hipError_t e = fill_u32(cnt, (uint64_t)T.n_multi * L.passes * kRadix, 0u, stream);
@@ -1260,11 +1277,17 @@ static hipError_t radix_segments_t(U* keys, void* vals_v, const SegTiles& T, con
 template <typename U, int VB>
 static hipError_t radix_segments_i(U* keys, void* vals, const SegTiles& T, const XformParams& xp, unsigned char* ws,
                                    const SegLayout& L, hipStream_t stream, LaunchTimer* timer, int policy) {
-  if (T.geo == 1) {
-    if (T.n_total < (1ull << 32))
-      return radix_segments_t<U, VB, uint32_t, 1>(keys, vals, T, xp, ws, L, stream, timer, policy);
-    return radix_segments_t<U, VB, uint64_t, 1>(keys, vals, T, xp, ws, L, stream, timer, policy);
-  }
+  const bool small_idx = T.n_total < (1ull << 32);
+  if (T.geo == 1)
+    return small_idx ? radix_segments_t<U, VB, uint32_t, 1>(keys, vals, T, xp, ws, L, stream, timer, policy)
+                     : radix_segments_t<U, VB, uint64_t, 1>(keys, vals, T, xp, ws, L, stream, timer, policy);
+  if (T.geo == 2)
+    return small_idx ? radix_segments_t<U, VB, uint32_t, 2>(keys, vals, T, xp, ws, L, stream, timer, policy)
+                     : radix_segments_t<U, VB, uint64_t, 2>(keys, vals, T, xp, ws, L, stream, timer, policy);
+  if (T.geo == 3)
+    return small_idx ? radix_segments_t<U, VB, uint32_t, 3>(keys, vals, T, xp, ws, L, stream, timer, policy)
+                     : radix_segments_t<U, VB, uint64_t, 3>(keys, vals, T, xp, ws, L, stream, timer, policy);
+  if (T.geo != 0) return hipErrorInvalidValue;
   if (T.n_total < (1ull << 32)) return radix_segments_t<U, VB, uint32_t, 0>(keys, vals, T, xp, ws, L, stream, timer, policy);
   return radix_segments_t<U, VB, uint64_t, 0>(keys, vals, T, xp, ws, L, stream, timer, policy);
 }

@@ -43,6 +43,7 @@ struct SegPlanHost {
   std::vector<uint32_t> small_ids, multi_ids[2], single_ids[2], huge;
   uint32_t small_len = 0;  // longest segment of the LDS class
   struct Group {
+    int geo = 0;  // tile geometry (radix_seg_tile_keys)
     uint32_t nslots = 0;
     uint64_t tiles = 0, n_keys = 0, tile_keys = 0;
     size_t off_tile_seg = 0, off_seg_first = 0, off_seg_range = 0;
@@ -52,15 +53,17 @@ struct SegPlanHost {
   size_t ws_bytes = 0;  // workspace of the radix groups
 };
 
-// Classifies the segments (radix groups by tile geometry g: 1 = small tiles
-// up to medium_max keys, 0 = large tiles; multi-tile segments first) and
-// lays out the maps.
+// Classifies the segments (radix groups g: 1 = small tiles, of the
+// configured medium geometry, up to medium_max keys; 0 = large tiles;
+// multi-tile segments first) and lays out the maps.
 void build_seg_plan(int kb, int vb, const uint64_t* off, uint32_t nseg, const SegConfig& cfg, SegPlanHost* P) {
   P->kb = kb;
   P->vb = vb;
   P->nseg = nseg;
   P->n_total = off[nseg];
   const uint32_t small_max = std::min<uint32_t>(cfg.small_max, block_sort_max(kb, vb));
+  P->grp[0].geo = 0;
+  P->grp[1].geo = 1 + (int)std::min<uint32_t>(cfg.medium_geometry, 2u);
   for (uint32_t s = 0; s < nseg; ++s) {
     const uint64_t len = off[s + 1] - off[s];
     if (len <= 1) continue;
@@ -71,13 +74,13 @@ void build_seg_plan(int kb, int vb, const uint64_t* off, uint32_t nseg, const Se
       P->huge.push_back(s);
     } else {
       const int g = len <= cfg.medium_max ? 1 : 0;
-      (len > radix_seg_tile_keys(kb, vb, g) ? P->multi_ids[g] : P->single_ids[g]).push_back(s);
+      (len > radix_seg_tile_keys(kb, vb, P->grp[g].geo) ? P->multi_ids[g] : P->single_ids[g]).push_back(s);
     }
   }
   size_t o = 0;
   for (int g = 0; g < 2; ++g) {
     SegPlanHost::Group& G = P->grp[g];
-    G.tile_keys = radix_seg_tile_keys(kb, vb, g);
+    G.tile_keys = radix_seg_tile_keys(kb, vb, G.geo);
     G.nslots = (uint32_t)(P->multi_ids[g].size() + P->single_ids[g].size());
     for (uint32_t s : P->multi_ids[g]) G.tiles += (off[s + 1] - off[s] + G.tile_keys - 1) / G.tile_keys;
     G.tiles += P->single_ids[g].size();
@@ -140,7 +143,7 @@ hipError_t launch_seg_plan(const SegPlanHost& P, const unsigned char* dm, void* 
     const SegPlanHost::Group& G = P.grp[g];
     if (!G.nslots) continue;
     SegTiles T;
-    T.geo = g;
+    T.geo = G.geo;
     T.tile_seg = reinterpret_cast<const uint32_t*>(dm + G.off_tile_seg);
     T.seg_first = reinterpret_cast<const uint32_t*>(dm + G.off_seg_first);
     T.seg_range = reinterpret_cast<const uint64_t*>(dm + G.off_seg_range);
@@ -247,6 +250,7 @@ gs_status gs_get_seg_config(gs_handle* h, gs_dtype kdt, gs_dtype vdt, gs_seg_con
   const SegConfig c = h->seg_config(dtype_bytes(kdt), val_bytes_of(vdt));
   out->small_max = c.small_max;
   out->medium_max = c.medium_max;
+  out->medium_geometry = c.medium_geometry;
   return GS_OK;
 }
 
@@ -256,7 +260,8 @@ gs_status gs_set_seg_config(gs_handle* h, gs_dtype kdt, gs_dtype vdt, const gs_s
   const int kb = dtype_bytes(kdt), vb = val_bytes_of(vdt);
   if (cfg->small_max > block_sort_max(kb, vb))
     return fail(GS_INVALID_ARG, "small_max %u exceeds the LDS tile of %u keys", cfg->small_max, block_sort_max(kb, vb));
-  h->seg_tuned[kb * 16 + vb] = SegConfig{cfg->small_max, cfg->medium_max};
+  if (cfg->medium_geometry > 2) return fail(GS_INVALID_ARG, "medium_geometry %u is not 0, 1 or 2", cfg->medium_geometry);
+  h->seg_tuned[kb * 16 + vb] = SegConfig{cfg->small_max, cfg->medium_max, cfg->medium_geometry};
   return GS_OK;
 }
 
@@ -407,7 +412,8 @@ extern "C" gs_status gs_autotune_segments(gs_handle* h, gs_dtype kdt, gs_dtype v
   const uint64_t t1 = radix_seg_tile_keys(kb, vb, 1);
   std::vector<SegConfig> cands;
   for (uint32_t sm : {cap, cap / 2, cap / 4})
-    for (uint64_t md : {(uint64_t)0, 8 * t1, 64 * t1}) cands.push_back(SegConfig{sm, md});
+    for (uint64_t md : {(uint64_t)0, 8 * t1, 64 * t1})
+      for (uint32_t geo = 0; geo < (md ? 3u : 1u); ++geo) cands.push_back(SegConfig{sm, md, geo});
   const XformParams xf = make_xform((int)kdt, false);
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   e = hipEventCreate(&ev0);
@@ -451,9 +457,10 @@ extern "C" gs_status gs_autotune_segments(gs_handle* h, gs_dtype kdt, gs_dtype v
   if (best) {
     best->small_max = cands[bi].small_max;
     best->medium_max = cands[bi].medium_max;
+    best->medium_geometry = cands[bi].medium_geometry;
   }
   if (n_rows) *n_rows = (uint32_t)std::min<size_t>(cands.size(), max_rows);
   for (size_t c = 0; rows && c < cands.size() && c < max_rows; ++c)
-    rows[c] = gs_tune_row{gs_seg_config{cands[c].small_max, cands[c].medium_max}, times[c]};
+    rows[c] = gs_tune_row{gs_seg_config{cands[c].small_max, cands[c].medium_max, cands[c].medium_geometry}, times[c]};
   return GS_OK;
 }

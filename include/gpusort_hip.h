@@ -235,6 +235,10 @@ gs_status gs_profile_end(gs_handle* h, gs_kernel_stat* out, uint32_t max_out, ui
 typedef struct gs_seg_config {
   uint32_t small_max;
   uint64_t medium_max;
+  /* Tile geometry of the small-tile class: 0 = 256 threads x 16 keys (x 12
+   * when a key and payload take more than 8 bytes), 1 = 256 x 8 (x 6),
+   * 2 = 512 x 16 (x 12). */
+  uint32_t medium_geometry;
 } gs_seg_config;
 gs_status gs_get_seg_config(gs_handle* h, gs_dtype kdt, gs_dtype vdt, gs_seg_config* out);
 gs_status gs_set_seg_config(gs_handle* h, gs_dtype kdt, gs_dtype vdt, const gs_seg_config* cfg);
@@ -248,9 +252,10 @@ typedef struct gs_tune_row {
 /* Autotuner of the batched sort: builds a synthetic batch of about
  * total_keys keys in arrays with lognormal lengths (median, sigma; clipped to
  * [1, 2^22]), uniform keys and index payloads of the given types, times every
- * candidate size-class configuration (LDS-tile limit x small-tile limit) with
- * events, stores the fastest in the handle and returns it (and every
- * candidate's time in rows, up to max_rows).  Synchronous. */
+ * candidate size-class configuration (LDS-tile limit x small-tile limit x
+ * small-tile geometry) with events, stores the fastest in the handle and
+ * returns it (and every candidate's time in rows, up to max_rows).
+ * Synchronous. */
 gs_status gs_autotune_segments(gs_handle* h, gs_dtype kdt, gs_dtype vdt, uint64_t total_keys, double median,
                                double sigma, gs_seg_config* best, gs_tune_row* rows, uint32_t max_rows,
                                uint32_t* n_rows);

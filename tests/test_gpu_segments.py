@@ -115,16 +115,20 @@ def test_repeat_calls_shrinking(handle):
     run_case(handle, np.uint32, np.uint32, [17000, 3], 28, descending=False)
 
 
-U32 = 1
+U32, U64, NONE = 1, 4, 255
 
 
-@pytest.mark.parametrize("small_max,medium_max", [(4096, 0), (1024, 1 << 16), (256, 1 << 20)],
-                         ids=["default", "small-tiles-64K", "small-tiles-1M"])
-def test_size_class_configs_agree(handle, small_max, medium_max):
-    # Every size-class configuration sorts identically.
-    handle.set_seg_config(U32, U32, small_max, medium_max)
+@pytest.mark.parametrize("small_max,medium_max,geometry",
+                         [(4096, 0, 0), (1024, 1 << 16, 0), (1024, 1 << 16, 1), (1024, 1 << 16, 2),
+                          (256, 1 << 20, 1), (256, 1 << 20, 2)],
+                         ids=["default", "small-tiles-64K", "half-tiles-64K", "wide-tiles-64K", "half-tiles-1M",
+                              "wide-tiles-1M"])
+def test_size_class_configs_agree(handle, small_max, medium_max, geometry):
+    # Every size-class configuration and small-tile geometry sorts identically.
+    handle.set_seg_config(U32, U32, small_max, medium_max, geometry)
     try:
-        assert handle.seg_config(U32, U32) == {"small_max": small_max, "medium_max": medium_max}
+        assert handle.seg_config(U32, U32) == {"small_max": small_max, "medium_max": medium_max,
+                                               "medium_geometry": geometry}
         run_case(handle, np.uint32, np.uint32, BOUNDARY_LENGTHS, 31, descending=False)
         lengths = datagen.lognormal_lengths(7, 2000, median=2048, sigma=2.0, lo=1, hi=1 << 20)
         run_case(handle, np.uint32, np.uint32, lengths, 32, descending=True)
@@ -132,13 +136,31 @@ def test_size_class_configs_agree(handle, small_max, medium_max):
         handle.set_seg_config(U32, U32, 4096, 0)
 
 
+# The small-tile geometries of the other widths (x 12 and x 6 keys per
+# thread when a key and payload take more than 8 bytes; keys alone).
+@pytest.mark.parametrize("kdt,vdt,kcode,vcode", [(np.uint64, np.uint64, U64, U64), (np.uint64, np.uint32, U64, U32),
+                                                 (np.uint32, None, U32, NONE)],
+                         ids=["u64+u64", "u64+u32", "u32"])
+@pytest.mark.parametrize("geometry", [1, 2])
+def test_small_tile_geometries_every_width(handle, kdt, vdt, kcode, vcode, geometry):
+    before = handle.seg_config(kcode, vcode)
+    handle.set_seg_config(kcode, vcode, 512, 1 << 16, geometry)
+    try:
+        lengths = datagen.lognormal_lengths(11, 1200, median=2048, sigma=2.0, lo=1, hi=1 << 18)
+        run_case(handle, kdt, vdt, lengths, 35 + geometry, descending=geometry == 2)
+    finally:
+        handle.set_seg_config(kcode, vcode, before["small_max"], before["medium_max"], before["medium_geometry"])
+
+
 def test_autotune_picks_the_fastest_candidate(handle):
     say("autotune u32+u32 on 2^22 keys")
     best, table = handle.autotune_segments(U32, U32, total_keys=1 << 22)
     try:
-        assert len(table) == 9 and all(r["ms"] > 0 for r in table)
+        # 3 LDS-tile limits x (no small-tile class, or 2 limits x 3 geometries)
+        assert len(table) == 21 and all(r["ms"] > 0 for r in table)
         fastest = min(table, key=lambda r: r["ms"])
-        assert best == {"small_max": fastest["small_max"], "medium_max": fastest["medium_max"]}
+        assert best == {"small_max": fastest["small_max"], "medium_max": fastest["medium_max"],
+                        "medium_geometry": fastest["medium_geometry"]}
         assert handle.seg_config(U32, U32) == best
         lengths = datagen.lognormal_lengths(8, 1500, median=2048, sigma=2.0, lo=1, hi=1 << 20)
         run_case(handle, np.uint32, np.uint32, lengths, 33, descending=False)
