@@ -231,3 +231,47 @@ def test_sort_segments(handle, kv):
         assert np.array_equal(gk[a:b], keys[a:b][perm]), f"segment {s}"
         if kv:
             assert np.array_equal(gv[a:b], vals[a:b][perm]), f"segment {s} payload"
+
+
+def _drawn_cases(seed, count):
+    """Seeded configurations beyond the hand-picked ones: algorithm, ranks
+    (powers of two and others), k, samples per rank, key type, payload,
+    direction and number of distinct keys, all drawn from one stream."""
+    d = datagen.draws(seed, 8 * count)
+    algos = ["hyksort", "samplesort", "hyperquicksort"]
+    dtypes = [np.uint32, np.int32, np.float32, np.uint64, np.int64, np.float64]
+    out = []
+    for c in range(count):
+        x = [int(v) for v in d[8 * c: 8 * c + 8]]
+        algo = algos[x[0] % 3]
+        p = [1, 2, 3, 4, 5, 8, 12, 16, 32, 64][x[1] % 10]
+        k = [2, 4, 8, 16][x[2] % 4] if algo == "hyksort" else 0
+        s = [1, 4, 16, 32][x[3] % 4] if algo != "samplesort" else 0
+        distinct = [None, None, 3, 50, 1000][x[4] % 5]
+        out.append((algo, p, k, s, dtypes[x[5] % 6], bool(x[6] & 1), bool(x[7] & 1), distinct))
+    return out
+
+
+@pytest.mark.parametrize("case", _drawn_cases(2024, 16),
+                         ids=lambda c: f"{c[0]}-p{c[1]}-k{c[2]}-s{c[3]}-{np.dtype(c[4]).name}"
+                         f"{'-kv' if c[5] else ''}{'-desc' if c[6] else ''}{'-d' + str(c[7]) if c[7] else ''}")
+def test_vrank_drawn_configurations(handle, case):
+    algo, p, k, s, dtype, kv, desc, distinct = case
+    off = rank_offsets(p, p * 17 + 3, 150 * p, 450 * p)
+    n = int(off[-1])
+    keys = keys_of(dtype, n, 5 * p + n, distinct=distinct)
+    vals = np.arange(n, dtype=np.uint32) if kv else None
+    say(f"vrank drawn {case} n={n}")
+    want_k, want_v, want = oracle.vrank(keys, off.astype(np.int64), vals, algorithm=algo, k_way=max(k, 2),
+                                        samples_per_rank=max(s, 1), descending=desc)
+    tk = dev(keys)
+    tv = dev(vals) if kv else None
+    got = handle.vrank_sort(tk, off, tv, algorithm=ALGOS[algo], k_way=max(k, 2), samples_per_rank=max(s, 1),
+                            descending=desc)
+    assert_bits_equal(host(tk), want_k, "keys")
+    if kv:
+        assert_bits_equal(host(tv), want_v, "payloads")
+    assert got["rounds"] == want["rounds"]
+    assert_bits_equal(got["splitters"], want["splitters"], "splitters")
+    assert np.array_equal(got["bucket_sizes"].astype(np.int64), want["bucket_sizes"]), "bucket sizes"
+    assert np.array_equal(got["final_rank_sizes"].astype(np.int64), want["final_rank_sizes"]), "final rank sizes"
