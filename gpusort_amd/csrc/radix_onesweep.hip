@@ -73,7 +73,8 @@ struct Vec16<uint64_t> {
   __device__ static uint64_t get(const T& v, int i) { return i == 0 ? v.x : v.y; }
 };
 
-// Digit histograms of every pass in one read of the keys.  Each lane adds
+// Digit histograms of every pass in one read of the keys (64-bit keys; the
+// 32-bit keys' kernel follows).  Each lane adds
 // into its own copy of the counters: copy c = lane % 32 sits in LDS bank c
 // (word = (pass * 128 + digit / 2) * 32 + c), so the 64 adds of a wave fall
 // two per bank whatever the digits, where one shared table takes random bank
@@ -83,7 +84,7 @@ struct Vec16<uint64_t> {
 constexpr int kHistThreads = 1024;
 constexpr uint64_t kHistKeysPerThread = 1024;
 
-template <typename U>
+template <typename U, bool IDENT>
 __global__ __launch_bounds__(kHistThreads) void radix_hist_kernel(const U* __restrict__ keys, uint64_t n, Xform<U> xf,
                                                                   int passes, unsigned long long* __restrict__ g_hist,
                                                                   uint4* __restrict__ zero_ptr, uint64_t zero_vec4) {
@@ -115,10 +116,10 @@ __global__ __launch_bounds__(kHistThreads) void radix_hist_kernel(const U* __res
   for (uint64_t vi = gtid; vi < nvec; vi += gstride) {
     const VT v = kv[vi];
 #pragma unroll
-    for (int e = 0; e < VN; ++e) count(to_ordered(Vec16<U>::get(v, e), xf));
+    for (int e = 0; e < VN; ++e) count(IDENT ? Vec16<U>::get(v, e) : to_ordered(Vec16<U>::get(v, e), xf));
   }
   if (blockIdx.x == 0)
-    for (uint64_t i = nvec * VN + tid; i < n; i += kHistThreads) count(to_ordered(keys[i], xf));
+    for (uint64_t i = nvec * VN + tid; i < n; i += kHistThreads) count(IDENT ? keys[i] : to_ordered(keys[i], xf));
   __syncthreads();
   // One thread per digit pair: the 32 copies summed, read in an order
   // rotated by thread so that a wave's reads spread over the banks.
@@ -133,6 +134,62 @@ __global__ __launch_bounds__(kHistThreads) void radix_hist_kernel(const U* __res
     const uint32_t p = w / (kRadix / 2), d = (w % (kRadix / 2)) * 2;
     if (lo) atomicAdd(&g_hist[p * kRadix + d], (unsigned long long)lo);
     if (hi) atomicAdd(&g_hist[p * kRadix + d + 1], (unsigned long long)hi);
+  }
+}
+
+// 32-bit keys: full 32-bit counters (no packing, so an add is a constant 1
+// at an address from one bit-field extract), 32 bank copies of the four
+// passes' 256 counters: 128 KB of LDS, one workgroup per CU, two 16-byte
+// loads in flight per thread.
+template <bool IDENT>
+__global__ __launch_bounds__(kHistThreads) void radix_hist32_kernel(const uint32_t* __restrict__ keys, uint64_t n,
+                                                                    Xform<uint32_t> xf,
+                                                                    unsigned long long* __restrict__ g_hist,
+                                                                    uint4* __restrict__ zero_ptr, uint64_t zero_vec4) {
+  constexpr uint32_t WORDS = 4 * kRadix * 32;
+  __shared__ __attribute__((aligned(16))) uint32_t s_cnt[WORDS];
+  const uint32_t tid = threadIdx.x;
+  const uint32_t copy = tid & 31u;
+  uint4* s4 = reinterpret_cast<uint4*>(s_cnt);
+  for (uint32_t i = tid; i < WORDS / 4; i += kHistThreads) s4[i] = make_uint4(0, 0, 0, 0);
+  const uint64_t gtid = (uint64_t)blockIdx.x * kHistThreads + tid;
+  const uint64_t gstride = (uint64_t)gridDim.x * kHistThreads;
+  for (uint64_t i = gtid; i < zero_vec4; i += gstride) zero_ptr[i] = make_uint4(0, 0, 0, 0);
+  __syncthreads();
+  auto count = [&](uint32_t u) {
+    if (!IDENT) u = to_ordered(u, xf);
+#pragma unroll
+    for (int p = 0; p < 4; ++p) atomicAdd(&s_cnt[(p * kRadix + ((u >> (8 * p)) & 0xFFu)) * 32u + copy], 1u);
+  };
+  const uint64_t nvec = n / 4;
+  const uint4* kv = reinterpret_cast<const uint4*>(keys);
+  uint64_t vi = gtid;
+  for (; vi + gstride < nvec; vi += 2 * gstride) {
+    const uint4 a = kv[vi], b = kv[vi + gstride];
+    count(a.x);
+    count(a.y);
+    count(a.z);
+    count(a.w);
+    count(b.x);
+    count(b.y);
+    count(b.z);
+    count(b.w);
+  }
+  if (vi < nvec) {
+    const uint4 a = kv[vi];
+    count(a.x);
+    count(a.y);
+    count(a.z);
+    count(a.w);
+  }
+  if (blockIdx.x == 0)
+    for (uint64_t i = nvec * 4 + tid; i < n; i += kHistThreads) count(keys[i]);
+  __syncthreads();
+  for (uint32_t w = tid; w < 4 * kRadix; w += kHistThreads) {
+    uint32_t sum = 0;
+#pragma unroll 8
+    for (uint32_t c = 0; c < 32; ++c) sum += s_cnt[w * 32u + ((c + w) & 31u)];
+    if (sum) atomicAdd(&g_hist[w], (unsigned long long)sum);
   }
 }
 
@@ -1023,8 +1080,23 @@ static hipError_t radix_sort_t(U* keys, void* vals_v, uint64_t n, const XformPar
   const uint64_t hist_blocks = hist_blocks_for(n);
   // The histogram kernel counts every key byte (L.passes == sizeof(U)).
   if (timer) timer->begin("radix_hist", n * sizeof(U) + status_bytes, stream);
-  hipLaunchKernelGGL(radix_hist_kernel<U>, dim3((uint32_t)hist_blocks), dim3(kHistThreads), 0, stream, keys, n, xf,
-                     L.passes, hist, reinterpret_cast<uint4*>(status), status_bytes / 16);
+  if constexpr (sizeof(U) == 4) {
+    // One workgroup per CU at most (128 KB of LDS each); fewer for small n.
+    const uint32_t blocks32 = (uint32_t)std::min<uint64_t>(256, hist_blocks);
+    if (ident)
+      hipLaunchKernelGGL((radix_hist32_kernel<true>), dim3(blocks32), dim3(kHistThreads), 0, stream,
+                         reinterpret_cast<const uint32_t*>(keys), n, *reinterpret_cast<const Xform<uint32_t>*>(&xf),
+                         hist, reinterpret_cast<uint4*>(status), status_bytes / 16);
+    else
+      hipLaunchKernelGGL((radix_hist32_kernel<false>), dim3(blocks32), dim3(kHistThreads), 0, stream,
+                         reinterpret_cast<const uint32_t*>(keys), n, *reinterpret_cast<const Xform<uint32_t>*>(&xf),
+                         hist, reinterpret_cast<uint4*>(status), status_bytes / 16);
+  } else if (ident)
+    hipLaunchKernelGGL((radix_hist_kernel<U, true>), dim3((uint32_t)hist_blocks), dim3(kHistThreads), 0, stream, keys,
+                       n, xf, L.passes, hist, reinterpret_cast<uint4*>(status), status_bytes / 16);
+  else
+    hipLaunchKernelGGL((radix_hist_kernel<U, false>), dim3((uint32_t)hist_blocks), dim3(kHistThreads), 0, stream,
+                       keys, n, xf, L.passes, hist, reinterpret_cast<uint4*>(status), status_bytes / 16);
   if (timer) timer->end(stream);
   hipLaunchKernelGGL(radix_scan_kernel, dim3(1), dim3(256), 0, stream, hist, base, ctrl, L.passes, n);
   for (int p = 0; p < L.passes; ++p) {
