@@ -40,19 +40,6 @@ constexpr int kRadix = 256;
 // Predecessor status words read per look-back step.
 constexpr int kLookback = 4;
 
-template <typename U>
-__device__ inline void hist_add(uint32_t* h, uint32_t d) {
-  // Many lanes of a skewed input hit the same LDS word; when the whole wave
-  // carries one digit a single add replaces 64 serialised atomics.
-  const uint64_t active = __builtin_amdgcn_read_exec();
-  const uint32_t d0 = __builtin_amdgcn_readfirstlane(d);
-  if (ballot(d == d0) == active) {
-    if (count_below(active) == 0) atomicAdd(&h[d0], (uint32_t)__popcll(active));
-  } else {
-    atomicAdd(&h[d], 1u);
-  }
-}
-
 __global__ void fill_u32_kernel(uint32_t* p, uint64_t n, uint32_t v) {
   const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
   for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) p[i] = v;
@@ -1188,14 +1175,21 @@ struct SegHistArgs {
 
 // Digit counts of one tile for every pass, added into its segment's counts
 // (atomically for multi-tile segments, whose counts start at zero; stored
-// for single-tile ones); also zeroes the tile's look-back words.
-template <typename U>
+// for single-tile ones); also zeroes the tile's look-back words.  Counters
+// come in kSegHistCopies copies interleaved by lane (lane % 4), so lanes
+// with one digit (the zero high bytes of small 64-bit keys) add into 16-way
+// rather than 64-way conflicting words; a key that the whole wave holds
+// (runs of equal keys) is added once per digit by one lane.
+constexpr uint32_t kSegHistCopies = 4;
+
+template <typename U, bool IDENT>
 __global__ __launch_bounds__(256) void seg_hist_kernel(SegHistArgs<U> a) {
   constexpr int MAXP = sizeof(U);  // one 8-bit digit per key byte (passes == MAXP)
-  __shared__ uint32_t s_hist[MAXP * kRadix];
+  __shared__ uint32_t s_hist[MAXP * kRadix * kSegHistCopies];  // [pass][digit][copy]
   const uint32_t tid = threadIdx.x, tile = blockIdx.x;
+  const uint32_t copy = tid & (kSegHistCopies - 1);
   const int passes = a.passes;
-  for (uint32_t i = tid; i < (uint32_t)MAXP * kRadix; i += 256) s_hist[i] = 0;
+  for (uint32_t i = tid; i < (uint32_t)MAXP * kRadix * kSegHistCopies; i += 256) s_hist[i] = 0;
   for (int p = 0; p < passes; ++p) a.status[p * a.status_per_pass + (uint64_t)tile * kRadix + tid] = 0;
   const uint32_t slot = a.tile_seg[tile];
   const uint64_t start = a.seg_range[2 * slot] + (uint64_t)(tile - a.seg_first[slot]) * a.tile_keys;
@@ -1203,9 +1197,23 @@ __global__ __launch_bounds__(256) void seg_hist_kernel(SegHistArgs<U> a) {
   const uint64_t end = (start + a.tile_keys < seg_end) ? start + a.tile_keys : seg_end;
   __syncthreads();
   auto count = [&](U key) {
-    const U u = to_ordered(key, a.xf);
+    const U u = IDENT ? key : to_ordered(key, a.xf);
+    const uint64_t active = __builtin_amdgcn_read_exec();
+    U u0;
+    if constexpr (sizeof(U) == 8)
+      u0 = ((U)__builtin_amdgcn_readfirstlane((uint32_t)(u >> 32)) << 32) | __builtin_amdgcn_readfirstlane((uint32_t)u);
+    else
+      u0 = __builtin_amdgcn_readfirstlane(u);
+    if (ballot(u == u0) == active) {
+      if (count_below(active) == 0) {
 #pragma unroll
-    for (int p = 0; p < MAXP; ++p) hist_add<U>(s_hist + p * kRadix, digit_of(u, p * 8));
+        for (int p = 0; p < MAXP; ++p)
+          atomicAdd(&s_hist[(p * kRadix + digit_of(u, p * 8)) * kSegHistCopies], (uint32_t)__popcll(active));
+      }
+    } else {
+#pragma unroll
+      for (int p = 0; p < MAXP; ++p) atomicAdd(&s_hist[(p * kRadix + digit_of(u, p * 8)) * kSegHistCopies + copy], 1u);
+    }
   };
   // Four keys per thread in flight: the loads are issued before any is
   // counted.
@@ -1220,11 +1228,15 @@ __global__ __launch_bounds__(256) void seg_hist_kernel(SegHistArgs<U> a) {
   for (; i < end; i += 256) count(a.keys[i]);
   __syncthreads();
   uint32_t* dst = a.seg_cnt + (uint64_t)slot * passes * kRadix;
-  if (slot < a.n_multi) {
-    for (uint32_t i = tid; i < (uint32_t)passes * kRadix; i += 256)
-      if (s_hist[i]) atomicAdd(&dst[i], s_hist[i]);
-  } else {
-    for (uint32_t i = tid; i < (uint32_t)passes * kRadix; i += 256) dst[i] = s_hist[i];
+  for (uint32_t i = tid; i < (uint32_t)passes * kRadix; i += 256) {
+    uint32_t c = 0;
+#pragma unroll
+    for (uint32_t j = 0; j < kSegHistCopies; ++j) c += s_hist[i * kSegHistCopies + j];
+    if (slot < a.n_multi) {
+      if (c) atomicAdd(&dst[i], c);
+    } else {
+      dst[i] = c;
+    }
   }
 }
 
@@ -1311,7 +1323,10 @@ static hipError_t radix_segments_t(U* keys, void* vals_v, const SegTiles& T, con
   h.status = status;
   h.status_per_pass = status_per_pass;
   if (timer) timer->begin("seg_hist", T.n_keys * sizeof(U), stream);
-  hipLaunchKernelGGL(seg_hist_kernel<U>, dim3((uint32_t)T.tiles), dim3(256), 0, stream, h);
+  if (ident)
+    hipLaunchKernelGGL((seg_hist_kernel<U, true>), dim3((uint32_t)T.tiles), dim3(256), 0, stream, h);
+  else
+    hipLaunchKernelGGL((seg_hist_kernel<U, false>), dim3((uint32_t)T.tiles), dim3(256), 0, stream, h);
   if (timer) timer->end(stream);
   hipLaunchKernelGGL(seg_scan_kernel, dim3(T.nslots), dim3(256), 0, stream, cnt, L.passes, ctrl);
   for (int p = 0; p < L.passes; ++p) {
