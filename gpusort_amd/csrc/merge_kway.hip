@@ -313,69 +313,58 @@ __device__ inline uint32_t group_start(const uint32_t* s_start, uint32_t r, uint
   return s_start[r < m ? r : m];
 }
 
+// Stages a partition's pieces in LDS in list order (coalesced within each
+// piece; every thread's loads are issued before any is used).
 template <typename U, typename V, bool KV, bool LEX>
-__global__ __launch_bounds__(kKwayThreads) void kway_merge_kernel(const U* __restrict__ sk, const V* __restrict__ sv,
-                                                                  U* __restrict__ dk, V* __restrict__ dv,
-                                                                  const KwayPart* __restrict__ parts, Xform<U> xf,
-                                                                  Xform<V> vxf) {
+__device__ __forceinline__ void kway_stage(const U* __restrict__ sk, const V* __restrict__ sv, U* s_k, V* s_v,
+                                           const uint32_t* s_start, const uint64_t* s_src, uint32_t total,
+                                           uint32_t tid, const Xform<U>& xf, const Xform<V>& vxf) {
   constexpr int VT = kKwayItems;
-  // Elements (ordered key, payload: its ordered image with LEX) of the
-  // partition, merged level by level in place: every level's outputs are
-  // gathered in registers before any is written back.
-  // VT slack slots past the cap: a thread's steps past its group's end
-  // read at most VT - 1 slots beyond it.
-  __shared__ U s_k[kKwayCap + VT];
-  __shared__ V s_v[KV ? kKwayCap + VT : 1];
-  __shared__ KwayPart s_part;
-
-  const uint32_t tid = threadIdx.x;
-  // The partition's record: 32 lanes, one dword each.
-  if (tid < sizeof(KwayPart) / 4)
-    reinterpret_cast<uint32_t*>(&s_part)[tid] = reinterpret_cast<const uint32_t*>(parts + blockIdx.x)[tid];
-  __syncthreads();
-  const uint32_t* const s_start = s_part.start;
-  const uint64_t* const s_src = s_part.src;
-  const uint32_t total = s_start[kKwayMax];
-  if (total == 0) return;  // past the pass's partitions (grid sized by an upper bound)
-  const uint32_t m = s_part.m;
-  const uint64_t out_base = s_part.out;
-
-  // Stage the pieces in list order (coalesced within each piece; every
-  // thread's loads are issued before any is used).
-  {
-    U lk[VT];
-    V lv[KV ? VT : 1];
+  U lk[VT];
+  V lv[KV ? VT : 1];
 #pragma unroll
-    for (int u = 0; u < VT; ++u) {
-      const uint32_t e = tid + (uint32_t)u * kKwayThreads;
-      if (e < total) {
-        uint32_t r = 0;
+  for (int u = 0; u < VT; ++u) {
+    const uint32_t e = tid + (uint32_t)u * kKwayThreads;
+    if (e < total) {
+      uint32_t r = 0;
 #pragma unroll
-        for (uint32_t c = 1; c < kKwayMax; ++c) r += e >= s_start[c] ? 1u : 0u;
-        const uint64_t a = s_src[r] + e;
-        lk[u] = sk[a];
-        if constexpr (KV) lv[u] = sv[a];
-      }
-    }
-#pragma unroll
-    for (int u = 0; u < VT; ++u) {
-      const uint32_t e = tid + (uint32_t)u * kKwayThreads;
-      if (e < total) {
-        s_k[e] = to_ordered(lk[u], xf);
-        if constexpr (KV) s_v[e] = LEX ? to_ordered(lv[u], vxf) : lv[u];
-      }
+      for (uint32_t c = 1; c < kKwayMax; ++c) r += e >= s_start[c] ? 1u : 0u;
+      const uint64_t a = s_src[r] + e;
+      lk[u] = sk[a];
+      if constexpr (KV) lv[u] = sv[a];
     }
   }
-  __syncthreads();
+#pragma unroll
+  for (int u = 0; u < VT; ++u) {
+    const uint32_t e = tid + (uint32_t)u * kKwayThreads;
+    if (e < total) {
+      s_k[e] = to_ordered(lk[u], xf);
+      if constexpr (KV) s_v[e] = LEX ? to_ordered(lv[u], vxf) : lv[u];
+    }
+  }
+}
 
-  // Merge tree over the pieces: at level L group j (pieces [j*2^L,
-  // (j+1)*2^L)) is the merge of its two halves, the earlier half winning
-  // ties.  Each group takes ceil(len / VT) threads in group order; a
-  // thread produces VT consecutive positions of its group (fewer at the
-  // group's end) with one merge-path search for its first position, then a
-  // serial merge of VT steps with no branch: a step past the group's end
-  // reads within the slack slots, and its output is not written back.
+// Merge tree over the staged pieces: at level L group j (pieces [j*2^L,
+// (j+1)*2^L)) is the merge of its two halves, the earlier half winning
+// ties.  Each group takes ceil(len / VT) threads in group order; a thread
+// produces VT consecutive positions of its group (fewer at the group's end)
+// with one merge-path search for its first position, then a serial merge of
+// VT steps with no branch: a step past the group's end reads within the
+// slack slots, and its output is not written back.  Every level's outputs
+// are gathered in registers before any is written back.
+//
+// TIES: equal keys compare their payloads (the (key, payload) order).
+// Without it equal keys go to the earlier half; with CHECK the function
+// then returns whether the merged partition has equal keys out of payload
+// order (block-wide), i.e. whether it differs from the (key, payload) merge:
+// a key-sorted sequence whose equal keys are in payload order is the
+// (key, payload) order of its elements.
+template <typename U, typename V, bool KV, bool TIES, bool CHECK>
+__device__ __forceinline__ bool kway_merge_tree(U* s_k, V* s_v, const uint32_t* s_start, uint32_t m, uint32_t tid) {
+  constexpr int VT = kKwayItems;
+  bool bad = false;
   for (uint32_t width = 1; width < m; width <<= 1) {
+    const bool last = 2 * width >= m;
     U rk[VT];
     V rv[VT];  // unused (and dropped) without payloads
     uint32_t g0 = 0, t0 = 0;
@@ -405,8 +394,9 @@ __global__ __launch_bounds__(kKwayThreads) void kway_merge_kernel(const U* __res
           const uint32_t mid = (lo + hi) >> 1;
           const uint32_t pa = a0 + mid, pb = b0 + diag - 1 - mid;
           const U xa = s_k[pa], xb = s_k[pb];
-          bool le = xa < xb;
-          if constexpr (LEX) {
+          bool le;
+          if constexpr (TIES) {
+            le = xa < xb;
             if (xa == xb) le = s_v[pa] <= s_v[pb];  // payloads only for equal keys
           } else {
             le = xa <= xb;
@@ -417,29 +407,44 @@ __global__ __launch_bounds__(kKwayThreads) void kway_merge_kernel(const U* __res
             hi = mid;
         }
       }
-      // Both heads are read at every step, at the sides' current indices,
-      // so no head is kept in registers; a side's index at or past its end
-      // reads a later slot (or a slack slot), ignored through the index.
-      uint32_t ia = a0 + lo, ib = b0 + (diag - lo);
+      // Both heads are read at every step through the sides' LDS pointers
+      // (32-bit LDS addresses), so no head is kept in registers; a side's
+      // pointer at or past its end reads a later slot (or a slack slot),
+      // ignored through the pointer.
+      using LU = __attribute__((address_space(3))) U;
+      using LV = __attribute__((address_space(3))) V;
+      LU* const lk = (LU*)s_k;
+      LV* const lv = (LV*)s_v;
+      LU* pa = lk + (a0 + lo);
+      LU* pb = lk + (b0 + (diag - lo));
+      LU* const ea = lk + b0;
+      LU* const eb = lk + b1;
 #pragma unroll
       for (int j = 0; j < VT; ++j) {
-        const U ka = s_k[ia], kb = s_k[ib];
-        V va = V(0), vb = V(0);
-        if constexpr (KV) {
-          va = s_v[ia];
-          vb = s_v[ib];
+        const U ka = *pa, kb = *pb;
+        if constexpr (TIES) {
+          const V va = lv[pa - lk], vb = lv[pb - lk];
+          const bool le = (ka < kb) | ((ka == kb) & (va <= vb));
+          const bool take_a = (pb >= eb) | ((pa < ea) & le);
+          rk[j] = take_a ? ka : kb;
+          rv[j] = take_a ? va : vb;
+          pa += take_a ? 1 : 0;
+          pb += take_a ? 0 : 1;
+        } else {
+          const bool take_a = (pb >= eb) | ((pa < ea) & (ka <= kb));
+          rk[j] = take_a ? ka : kb;
+          // the taken element's payload, read once through its key's slot
+          if constexpr (KV) rv[j] = lv[(take_a ? pa : pb) - lk];
+          pa += take_a ? 1 : 0;
+          pb += take_a ? 0 : 1;
         }
-        // first half on ties; bitwise operators keep the step branch-free
-        bool le;
-        if constexpr (LEX)
-          le = (ka < kb) | ((ka == kb) & (va <= vb));
-        else
-          le = ka <= kb;
-        const bool take_a = (ib >= b1) | ((ia < b0) & le);
-        rk[j] = take_a ? ka : kb;
-        if constexpr (KV) rv[j] = take_a ? va : vb;
-        ia += take_a ? 1u : 0u;
-        ib += take_a ? 0u : 1u;
+      }
+      if constexpr (CHECK) {
+        if (last) {
+#pragma unroll
+          for (int j = 1; j < VT; ++j)
+            bad |= ((uint32_t)j < cnt) & (rk[j] == rk[j - 1]) & (rv[j] < rv[j - 1]);
+        }
       }
     }
     __syncthreads();
@@ -451,6 +456,51 @@ __global__ __launch_bounds__(kKwayThreads) void kway_merge_kernel(const U* __res
       }
     }
     __syncthreads();
+    if constexpr (CHECK) {
+      // the pair across the boundary with the previous thread's outputs
+      if (last && cnt != 0 && o0 != 0) bad |= (s_k[o0 - 1] == rk[0]) & (s_v[o0 - 1] > rv[0]);
+    }
+  }
+  if constexpr (CHECK) return __syncthreads_or(bad) != 0;
+  return false;
+}
+
+template <typename U, typename V, bool KV, bool LEX>
+__global__ __launch_bounds__(kKwayThreads) void kway_merge_kernel(const U* __restrict__ sk, const V* __restrict__ sv,
+                                                                  U* __restrict__ dk, V* __restrict__ dv,
+                                                                  const KwayPart* __restrict__ parts, Xform<U> xf,
+                                                                  Xform<V> vxf) {
+  // Elements (ordered key, payload: its ordered image with LEX) of the
+  // partition.  VT slack slots past the cap: a thread's steps past its
+  // group's end read at most VT - 1 slots beyond it.
+  __shared__ U s_k[kKwayCap + kKwayItems];
+  __shared__ V s_v[KV ? kKwayCap + kKwayItems : 1];
+  __shared__ KwayPart s_part;
+
+  const uint32_t tid = threadIdx.x;
+  // The partition's record: 32 lanes, one dword each.
+  if (tid < sizeof(KwayPart) / 4)
+    reinterpret_cast<uint32_t*>(&s_part)[tid] = reinterpret_cast<const uint32_t*>(parts + blockIdx.x)[tid];
+  __syncthreads();
+  const uint32_t* const s_start = s_part.start;
+  const uint64_t* const s_src = s_part.src;
+  const uint32_t total = s_start[kKwayMax];
+  if (total == 0) return;  // past the pass's partitions (grid sized by an upper bound)
+  const uint32_t m = s_part.m;
+  const uint64_t out_base = s_part.out;
+
+  kway_stage<U, V, KV, LEX>(sk, sv, s_k, s_v, s_start, s_src, total, tid, xf, vxf);
+  __syncthreads();
+  if constexpr (LEX) {
+    // Keys first; merged again with payloads compared on equal keys only
+    // where equal keys of different pieces came out of payload order.
+    if (kway_merge_tree<U, V, KV, false, true>(s_k, s_v, s_start, m, tid)) {
+      kway_stage<U, V, KV, LEX>(sk, sv, s_k, s_v, s_start, s_src, total, tid, xf, vxf);
+      __syncthreads();
+      kway_merge_tree<U, V, KV, true, false>(s_k, s_v, s_start, m, tid);
+    }
+  } else {
+    kway_merge_tree<U, V, KV, false, false>(s_k, s_v, s_start, m, tid);
   }
   for (uint32_t o = tid; o < total; o += kKwayThreads) {
     dk[out_base + o] = from_ordered(s_k[o], xf);

@@ -168,6 +168,50 @@ def test_vrank_large_ranks(handle, case):
             assert got["max_rank_error"] <= 0.01 / (min(k, p) - 1)
 
 
+TIE_CASES = [
+    # algorithm, p, k, key dtype, payload dtype, descending, distinct keys, distinct payloads
+    ("hyksort", 16, 4, np.uint32, np.uint32, False, 3, None),
+    ("hyksort", 8, 8, np.uint64, np.uint32, True, 50, None),
+    ("hyksort", 8, 4, np.uint32, np.uint64, False, 1, None),
+    ("hyksort", 16, 2, np.uint64, np.int64, False, 5, 7),  # equal (key, payload) pairs
+    ("samplesort", 12, 0, np.int32, np.uint32, False, 4, None),
+    ("hyperquicksort", 8, 0, np.float32, np.uint64, True, 6, 3),
+]
+
+
+@pytest.mark.parametrize("case", TIE_CASES, ids=lambda c: f"{c[0]}-p{c[1]}-k{c[2]}-{np.dtype(c[3]).name}-"
+                         f"{np.dtype(c[4]).name}{'-desc' if c[5] else ''}-d{c[6]}{'-v' + str(c[7]) if c[7] else ''}")
+def test_vrank_random_payloads_on_tied_keys(handle, case):
+    """Few distinct keys with payloads in no particular order: every merge
+    meets equal keys whose payloads decide the order, across the pieces of
+    one partition as well as within them."""
+    algo, p, k, kdt, vdt, desc, distinct, vdistinct = case
+    off = rank_offsets(p, p * 7 + 11, 3000, 9000)
+    n = int(off[-1])
+    keys = keys_of(kdt, n, 3 * p + n, distinct=distinct)
+    r = datagen.draws(5 * n + p, n)
+    if vdistinct:
+        r = r % np.uint64(vdistinct)
+    vals = (r >> np.uint64(32)).astype(np.uint32) if np.dtype(vdt).itemsize == 4 and not vdistinct else r.astype(vdt)
+    say(f"vrank ties {case} n={n}")
+    want_k, want_v, want = oracle.vrank(keys, off.astype(np.int64), vals, algorithm=algo, k_way=max(k, 2),
+                                        samples_per_rank=16, descending=desc)
+    tk, tv = dev(keys), dev(vals)
+    got = handle.vrank_sort(tk, off, tv, algorithm=ALGOS[algo], k_way=max(k, 2), samples_per_rank=16,
+                            descending=desc)
+    assert_bits_equal(host(tk), want_k, "keys")
+    assert_bits_equal(host(tv), want_v, "payloads")
+    assert np.array_equal(got["final_rank_sizes"].astype(np.int64), want["final_rank_sizes"]), "final rank sizes"
+    assert_bits_equal(got["splitters"], want["splitters"], "splitters")
+    assert np.array_equal(got["bucket_sizes"].astype(np.int64), want["bucket_sizes"]), "bucket sizes"
+    # the concatenation is the (key, payload) order of the input
+    order = np.lexsort((vals, keys))
+    if desc:
+        order = order[::-1]
+    assert_bits_equal(want_k, keys[order], "oracle keys in (key, payload) order")
+    assert_bits_equal(want_v, vals[order], "oracle payloads in (key, payload) order")
+
+
 SMALL_CASES = [
     # algorithm, p, key dtype, payload, descending, distinct keys: N < 5 p^2 takes the bitonic network
     ("samplesort", 2, np.uint32, True, False, None),
