@@ -250,6 +250,9 @@ struct HykRun {
   int cur;
   uint32_t p, k_way, s;
   uint64_t n;
+  // Pinned host memory of at least hyk_host_bytes(): the selection summaries
+  // and the report come back through it (one copy each).
+  void* host;
 };
 // What the rounds leave for the report: per round, the bucket bounds
 // (block of p * bnd_stride words, rank r at r * (kway + 2): run start,
@@ -265,6 +268,7 @@ struct HykResult {
 };
 bool hyk_device_supported(uint32_t p, uint32_t k_way, uint32_t s);
 size_t hyk_device_workspace_bytes(uint32_t p, uint32_t k_way, uint32_t s, uint64_t n, int kb, int vb);
+size_t hyk_host_bytes(uint32_t p, uint32_t k_way, uint32_t s, int kb, int vb);
 hipError_t hyk_rounds_device(HykRun* run, const uint64_t* h_off, void* workspace, size_t workspace_bytes,
                              hipStream_t stream, LaunchTimer* timer, HykResult* res);
 

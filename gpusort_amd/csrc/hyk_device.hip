@@ -565,6 +565,7 @@ uint32_t pass_chunks(uint32_t kk, uint32_t t) {
 struct HykLayout {
   size_t off[2], grp, wstart, wend, cumw, cnt, sk, sv, glb, pick, spl_k, spl_v, summary, spl_raw_k, spl_raw_v,
       spl_first, bnd, iters, dests, totals, kscratch, loc, total;
+  size_t report;  // bytes of the leading region read back at the end (summary .. spl_raw_v)
   uint32_t rounds, max_cap, max_kway, max_chunks;
   bool keep_loc;  // room for every sample's bound in every rank
 };
@@ -615,8 +616,14 @@ HykLayout hyk_layout(uint32_t p, uint32_t k_way, uint32_t s, uint64_t n, int kb,
     o = align_up(o + (bytes ? bytes : 8), 256);
     return at;
   };
+  // What the host reads back comes first, in one region.
+  L.summary = take(16);
   L.off[0] = take((size_t)(p + 1) * 8);
   L.off[1] = take((size_t)(p + 1) * 8);
+  L.bnd = take((size_t)L.rounds * p * (L.max_kway + 2) * 8);
+  L.spl_raw_k = take((size_t)L.rounds * p * L.max_kway * kb);
+  L.spl_raw_v = take((size_t)L.rounds * p * L.max_kway * vbb);
+  L.report = o;
   L.grp = take((size_t)p * sizeof(HykGroup));
   L.wstart = take((size_t)p * L.max_kway * 8);
   L.wend = take((size_t)p * L.max_kway * 8);
@@ -628,11 +635,7 @@ HykLayout hyk_layout(uint32_t p, uint32_t k_way, uint32_t s, uint64_t n, int kb,
   L.pick = take((size_t)p * L.max_kway * sizeof(HykPick));
   L.spl_k = take((size_t)p * L.max_kway * kb);
   L.spl_v = take((size_t)p * L.max_kway * vbb);
-  L.summary = take(16);
-  L.spl_raw_k = take((size_t)L.rounds * p * L.max_kway * kb);
-  L.spl_raw_v = take((size_t)L.rounds * p * L.max_kway * vbb);
   L.spl_first = take((size_t)p * 4);
-  L.bnd = take((size_t)L.rounds * p * (L.max_kway + 2) * 8);
   L.iters = take((size_t)L.rounds * 4);
   L.dests = take((size_t)p * L.max_chunks * sizeof(KwayDest));
   L.totals = take(sizeof(KwayTotals));
@@ -665,7 +668,11 @@ hipError_t hyk_rounds_t(HykRun* run, const uint64_t* h_off, char* ws, const HykL
   const uint32_t p = run->p;
   uint64_t* off[2] = {reinterpret_cast<uint64_t*>(ws + L.off[0]), reinterpret_cast<uint64_t*>(ws + L.off[1])};
   int oc = 0;
-  hipError_t e = hipMemcpyAsync(off[0], h_off, (size_t)(p + 1) * 8, hipMemcpyHostToDevice, stream);
+  // the rank offsets go up through the pinned buffer (at their own offset
+  // in the report region, which only the final read-back overwrites)
+  std::memcpy(reinterpret_cast<char*>(run->host) + L.off[0], h_off, (size_t)(p + 1) * 8);
+  hipError_t e = hipMemcpyAsync(off[0], reinterpret_cast<char*>(run->host) + L.off[0], (size_t)(p + 1) * 8,
+                                hipMemcpyHostToDevice, stream);
   if (e != hipSuccess) return e;
   HykWs<U, V> W;
   W.grp = reinterpret_cast<HykGroup*>(ws + L.grp);
@@ -723,9 +730,10 @@ hipError_t hyk_rounds_t(HykRun* run, const uint64_t* h_off, char* ws, const HykL
       }
       launched += batch;
       hipLaunchKernelGGL(hyk_summary_kernel, dim3(1), dim3(kHykThreads), 0, stream, D, W.grp, W.summary);
-      e = hipMemcpyAsync(summary, W.summary, sizeof(summary), hipMemcpyDeviceToHost, stream);
+      e = hipMemcpyAsync(run->host, W.summary, sizeof(summary), hipMemcpyDeviceToHost, stream);
       if (e == hipSuccess) e = hipStreamSynchronize(stream);
       if (e != hipSuccess) return e;
+      std::memcpy(summary, run->host, sizeof(summary));
       if (summary[0]) break;
     }
     res->iterations[ri] = summary[1];
@@ -768,21 +776,25 @@ hipError_t hyk_rounds_t(HykRun* run, const uint64_t* h_off, char* ws, const HykL
   res->spl_v.assign((size_t)shapes.size() * p * L.max_kway * sizeof(V), 0);
   res->bnd_stride = L.max_kway + 2;
   res->spl_stride = L.max_kway;
-  e = hipMemcpyAsync(res->final_off.data(), off[oc], (size_t)(p + 1) * 8, hipMemcpyDeviceToHost, stream);
-  if (e == hipSuccess)
-    e = hipMemcpyAsync(res->bounds.data(), ws + L.bnd, res->bounds.size() * 8, hipMemcpyDeviceToHost, stream);
-  if (e == hipSuccess)
-    e = hipMemcpyAsync(res->spl_k.data(), ws + L.spl_raw_k, res->spl_k.size(), hipMemcpyDeviceToHost, stream);
-  if (e == hipSuccess && LEX)
-    e = hipMemcpyAsync(res->spl_v.data(), ws + L.spl_raw_v, res->spl_v.size(), hipMemcpyDeviceToHost, stream);
+  e = hipMemcpyAsync(run->host, ws, L.report, hipMemcpyDeviceToHost, stream);
   if (e == hipSuccess) e = hipStreamSynchronize(stream);
-  return e;
+  if (e != hipSuccess) return e;
+  const char* h = reinterpret_cast<const char*>(run->host);
+  std::memcpy(res->final_off.data(), h + L.off[oc], res->final_off.size() * 8);
+  std::memcpy(res->bounds.data(), h + L.bnd, res->bounds.size() * 8);
+  std::memcpy(res->spl_k.data(), h + L.spl_raw_k, res->spl_k.size());
+  if (LEX) std::memcpy(res->spl_v.data(), h + L.spl_raw_v, res->spl_v.size());
+  return hipSuccess;
 }
 
 }  // namespace
 
 size_t hyk_device_workspace_bytes(uint32_t p, uint32_t k_way, uint32_t s, uint64_t n, int kb, int vb) {
   return hyk_layout(p, k_way, s, n, kb, vb).total;
+}
+
+size_t hyk_host_bytes(uint32_t p, uint32_t k_way, uint32_t s, int kb, int vb) {
+  return hyk_layout(p, k_way, s, 0, kb, vb).report;
 }
 
 bool hyk_device_supported(uint32_t p, uint32_t k_way, uint32_t s) {

@@ -363,6 +363,24 @@ void regular_samples(uint64_t o, uint64_t len, uint32_t s, std::vector<uint64_t>
   for (uint32_t j = 0; j < s; ++j) idx->push_back(o + (uint64_t)(j + 1) * len / (s + 1));
 }
 
+// The handle's pinned host buffer, grown to at least `bytes` (every call
+// that uses it synchronises before returning, so it is free to replace).
+hipError_t host_pinned(gs_handle* h, size_t bytes, void** out) {
+  if (h->pinned_bytes < bytes) {
+    if (h->pinned) (void)hipHostFree(h->pinned);
+    h->pinned = nullptr;
+    h->pinned_bytes = 0;
+    hipError_t e = hipHostMalloc(&h->pinned, bytes, hipHostMallocDefault);
+    if (e != hipSuccess) {
+      h->pinned = nullptr;
+      return e;
+    }
+    h->pinned_bytes = bytes;
+  }
+  *out = h->pinned;
+  return hipSuccess;
+}
+
 // HykSort's rounds, every step on the device (hyk_device.hip); the report
 // is filled from the bounds and splitters the rounds leave behind.
 gs_status hyksort_rounds(Pipe& P, std::vector<uint64_t>& sizes, uint32_t k_way, uint32_t s, Report& rep,
@@ -375,8 +393,12 @@ gs_status hyksort_rounds(Pipe& P, std::vector<uint64_t>& sizes, uint32_t k_way, 
   void* ws = nullptr;
   hipError_t e = P.h->arena.get(Arena::kHykWork, ws_bytes, P.h->stream, &ws);
   if (e != hipSuccess) return from_hip(e, "HykSort workspace");
+  void* host = nullptr;
+  e = host_pinned(P.h, hyk_host_bytes(p, k_way, s, P.kb, P.kv ? P.vb : 0), &host);
+  if (e != hipSuccess) return from_hip(e, "HykSort host buffer");
   const std::vector<uint64_t> off = offsets_of(sizes);
   HykRun run{};
+  run.host = host;
   run.kb = P.kb;
   run.vb = P.kv ? P.vb : 0;
   run.lex = P.kv;
@@ -786,13 +808,14 @@ extern "C" gs_status gs_vrank_sort(gs_handle* h, gs_algorithm algo, gs_dtype kdt
     void* ws = nullptr;
     e = h->arena.get(Arena::kPipeAux, 256, h->stream, &ws);
     if (e != hipSuccess) return from_hip(e, "pipeline workspace");
-    uint32_t flag = 0;
-    e = hipMemcpyAsync(ws, &flag, 4, hipMemcpyHostToDevice, h->stream);
+    void* host = nullptr;
+    e = host_pinned(h, 256, &host);
+    if (e == hipSuccess) e = fill_u32(reinterpret_cast<uint32_t*>(ws), 1, 0u, h->stream);
     if (e == hipSuccess) e = unsorted_flag(P.vb, d_vals, n, P.vx, reinterpret_cast<uint32_t*>(ws), h->stream);
-    if (e == hipSuccess) e = hipMemcpyAsync(&flag, ws, 4, hipMemcpyDeviceToHost, h->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(host, ws, 4, hipMemcpyDeviceToHost, h->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
     if (e != hipSuccess) return from_hip(e, "payload order check");
-    lex_values = flag != 0;
+    lex_values = *reinterpret_cast<const uint32_t*>(host) != 0;
   }
   {
     gs_status st = sort_runs(P, P.kbuf[0], P.kv ? P.vbuf[0] : nullptr,
