@@ -307,12 +307,6 @@ __global__ __launch_bounds__(256) void kway_part_kernel(const KwayDest* __restri
   parts[x] = P;
 }
 
-// Slots of piece group [r0, r1) of a partition: pieces r0..r1-1 (clamped
-// to m) are contiguous in the staging buffer.
-__device__ inline uint32_t group_start(const uint32_t* s_start, uint32_t r, uint32_t m) {
-  return s_start[r < m ? r : m];
-}
-
 // Stages a partition's pieces in LDS in list order (coalesced within each
 // piece; every thread's loads are issued before any is used).
 template <typename U, typename V, bool KV, bool LEX>
@@ -359,108 +353,137 @@ __device__ __forceinline__ void kway_stage(const U* __restrict__ sk, const V* __
 // order (block-wide), i.e. whether it differs from the (key, payload) merge:
 // a key-sorted sequence whose equal keys are in payload order is the
 // (key, payload) order of its elements.
-template <typename U, typename V, bool KV, bool TIES, bool CHECK>
-__device__ __forceinline__ bool kway_merge_tree(U* s_k, V* s_v, const uint32_t* s_start, uint32_t m, uint32_t tid) {
+// One level of the merge tree (groups of 2 * WIDTH pieces); returns, with
+// CHECK on the last level, whether this thread saw equal keys out of
+// payload order.  Piece starts at or past m equal the total, so group g
+// spans st[g] .. st[g + 2 * WIDTH] (indices <= 8) and groups past m are
+// empty.
+template <typename U, typename V, bool KV, bool TIES, bool CHECK, uint32_t WIDTH>
+__device__ __forceinline__ bool kway_level(U* s_k, V* s_v, const uint32_t (&st)[kKwayMax + 1], bool last,
+                                           uint32_t tid) {
   constexpr int VT = kKwayItems;
+  using LU = __attribute__((address_space(3))) U;
+  using LV = __attribute__((address_space(3))) V;
+  LU* const lk = (LU*)s_k;
+  LV* const lv = (LV*)s_v;
   bool bad = false;
-  for (uint32_t width = 1; width < m; width <<= 1) {
-    const bool last = 2 * width >= m;
-    U rk[VT];
-    V rv[VT];  // unused (and dropped) without payloads
-    uint32_t g0 = 0, t0 = 0;
-    bool mine = false;
-    for (uint32_t g = 0; g < m; g += 2 * width) {
-      const uint32_t len = group_start(s_start, g + 2 * width, m) - group_start(s_start, g, m);
-      const uint32_t nt = (len + VT - 1) / VT;
-      if (!mine && tid < t0 + nt) {
-        mine = true;
-        g0 = g;
-      }
-      if (!mine) t0 += nt;
-    }
-    uint32_t o0 = 0, cnt = 0;
-    if (mine) {
-      const uint32_t a0 = group_start(s_start, g0, m);
-      const uint32_t b0 = group_start(s_start, g0 + width, m);
-      const uint32_t b1 = group_start(s_start, g0 + 2 * width, m);
-      const uint32_t diag = (tid - t0) * VT;
-      o0 = a0 + diag;
-      cnt = b1 - o0 < (uint32_t)VT ? b1 - o0 : (uint32_t)VT;
-      const uint32_t na = b0 - a0, nb = b1 - b0;
-      uint32_t lo = diag > nb ? diag - nb : 0;
-      {
-        uint32_t hi = diag < na ? diag : na;
-        while (lo < hi) {
-          const uint32_t mid = (lo + hi) >> 1;
-          const uint32_t pa = a0 + mid, pb = b0 + diag - 1 - mid;
-          const U xa = s_k[pa], xb = s_k[pb];
-          bool le;
-          if constexpr (TIES) {
-            le = xa < xb;
-            if (xa == xb) le = s_v[pa] <= s_v[pb];  // payloads only for equal keys
-          } else {
-            le = xa <= xb;
-          }
-          if (le)
-            lo = mid + 1;
-          else
-            hi = mid;
-        }
-      }
-      // Both heads are read at every step through the sides' LDS pointers
-      // (32-bit LDS addresses), so no head is kept in registers; a side's
-      // pointer at or past its end reads a later slot (or a slack slot),
-      // ignored through the pointer.
-      using LU = __attribute__((address_space(3))) U;
-      using LV = __attribute__((address_space(3))) V;
-      LU* const lk = (LU*)s_k;
-      LV* const lv = (LV*)s_v;
-      LU* pa = lk + (a0 + lo);
-      LU* pb = lk + (b0 + (diag - lo));
-      LU* const ea = lk + b0;
-      LU* const eb = lk + b1;
+  U rk[VT];
+  V rv[VT];  // unused (and dropped) without payloads
+  uint32_t a0 = 0, b0 = 0, b1 = 0, t0 = 0, tb = 0;
+  bool mine = false;
 #pragma unroll
-      for (int j = 0; j < VT; ++j) {
-        const U ka = *pa, kb = *pb;
+  for (uint32_t g = 0; g < kKwayMax; g += 2 * WIDTH) {
+    const uint32_t nt = (st[g + 2 * WIDTH] - st[g] + VT - 1) / VT;
+    const bool in = (tid >= tb) & (tid < tb + nt);
+    a0 = in ? st[g] : a0;
+    b0 = in ? st[g + WIDTH] : b0;
+    b1 = in ? st[g + 2 * WIDTH] : b1;
+    t0 = in ? tb : t0;
+    mine |= in;
+    tb += nt;
+  }
+  uint32_t o0 = 0, cnt = 0;
+  if (mine) {
+    const uint32_t diag = (tid - t0) * VT;
+    o0 = a0 + diag;
+    cnt = b1 - o0 < (uint32_t)VT ? b1 - o0 : (uint32_t)VT;
+    const uint32_t na = b0 - a0, nb = b1 - b0;
+    uint32_t lo = diag > nb ? diag - nb : 0;
+    {
+      uint32_t hi = diag < na ? diag : na;
+      LU* const sa = lk + a0;
+      LU* const sb = lk + (b0 + diag - 1);
+      while (lo < hi) {
+        const uint32_t mid = (lo + hi) >> 1;
+        const U xa = sa[mid], xb = *(sb - mid);
+        bool le;
         if constexpr (TIES) {
-          const V va = lv[pa - lk], vb = lv[pb - lk];
-          const bool le = (ka < kb) | ((ka == kb) & (va <= vb));
-          const bool take_a = (pb >= eb) | ((pa < ea) & le);
-          rk[j] = take_a ? ka : kb;
-          rv[j] = take_a ? va : vb;
-          pa += take_a ? 1 : 0;
-          pb += take_a ? 0 : 1;
+          le = xa < xb;
+          if (xa == xb) le = lv[(sa + mid) - lk] <= lv[(sb - mid) - lk];  // payloads only for equal keys
         } else {
-          const bool take_a = (pb >= eb) | ((pa < ea) & (ka <= kb));
-          rk[j] = take_a ? ka : kb;
-          // the taken element's payload, read once through its key's slot
-          if constexpr (KV) rv[j] = lv[(take_a ? pa : pb) - lk];
-          pa += take_a ? 1 : 0;
-          pb += take_a ? 0 : 1;
+          le = xa <= xb;
         }
-      }
-      if constexpr (CHECK) {
-        if (last) {
-#pragma unroll
-          for (int j = 1; j < VT; ++j)
-            bad |= ((uint32_t)j < cnt) & (rk[j] == rk[j - 1]) & (rv[j] < rv[j - 1]);
-        }
+        if (le)
+          lo = mid + 1;
+        else
+          hi = mid;
       }
     }
-    __syncthreads();
+    // Both heads are read at every step through the sides' LDS pointers
+    // (32-bit LDS addresses), so no head is kept in registers; a side's
+    // pointer at or past its end reads a later slot (or a slack slot),
+    // ignored through the pointer.
+    LU* pa = lk + (a0 + lo);
+    LU* pb = lk + (b0 + (diag - lo));
+    LU* const ea = lk + b0;
+    LU* const eb = lk + b1;
 #pragma unroll
     for (int j = 0; j < VT; ++j) {
-      if ((uint32_t)j < cnt) {
-        s_k[o0 + j] = rk[j];
-        if constexpr (KV) s_v[o0 + j] = rv[j];
+      const U ka = *pa, kb = *pb;
+      if constexpr (TIES) {
+        const V va = lv[pa - lk], vb = lv[pb - lk];
+        const bool le = (ka < kb) | ((ka == kb) & (va <= vb));
+        const bool take_a = (pb >= eb) | ((pa < ea) & le);
+        rk[j] = take_a ? ka : kb;
+        rv[j] = take_a ? va : vb;
+        pa += take_a ? 1 : 0;
+        pb += take_a ? 0 : 1;
+      } else {
+        const bool take_a = (pb >= eb) | ((pa < ea) & (ka <= kb));
+        rk[j] = take_a ? ka : kb;
+        // the taken element's payload, read once through its key's slot
+        if constexpr (KV) rv[j] = lv[(take_a ? pa : pb) - lk];
+        pa += take_a ? 1 : 0;
+        pb += take_a ? 0 : 1;
       }
     }
-    __syncthreads();
     if constexpr (CHECK) {
-      // the pair across the boundary with the previous thread's outputs
-      if (last && cnt != 0 && o0 != 0) bad |= (s_k[o0 - 1] == rk[0]) & (s_v[o0 - 1] > rv[0]);
+      if (last) {
+#pragma unroll
+        for (int j = 1; j < VT; ++j)
+          bad |= ((uint32_t)j < cnt) & (rk[j] == rk[j - 1]) & (rv[j] < rv[j - 1]);
+      }
     }
   }
+  __syncthreads();
+#pragma unroll
+  for (int j = 0; j < VT; ++j) {
+    if ((uint32_t)j < cnt) {
+      s_k[o0 + j] = rk[j];
+      if constexpr (KV) s_v[o0 + j] = rv[j];
+    }
+  }
+  __syncthreads();
+  if constexpr (CHECK) {
+    // the pair across the boundary with the previous thread's outputs
+    if (last && cnt != 0 && o0 != 0) bad |= (s_k[o0 - 1] == rk[0]) & (s_v[o0 - 1] > rv[0]);
+  }
+  return bad;
+}
+
+// Merge tree over the staged pieces: at level L group j (pieces [j*2^L,
+// (j+1)*2^L)) is the merge of its two halves, the earlier half winning
+// ties.  Each group takes ceil(len / VT) threads in group order; a thread
+// produces VT consecutive positions of its group (fewer at the group's end)
+// with one merge-path search for its first position, then a serial merge of
+// VT steps with no branch: a step past the group's end reads within the
+// slack slots, and its output is not written back.  Every level's outputs
+// are gathered in registers before any is written back.
+//
+// TIES: equal keys compare their payloads (the (key, payload) order).
+// Without it equal keys go to the earlier half; with CHECK the function
+// then returns whether the merged partition has equal keys out of payload
+// order (block-wide), i.e. whether it differs from the (key, payload) merge:
+// a key-sorted sequence whose equal keys are in payload order is the
+// (key, payload) order of its elements.
+template <typename U, typename V, bool KV, bool TIES, bool CHECK>
+__device__ __forceinline__ bool kway_merge_tree(U* s_k, V* s_v, const uint32_t (&st)[kKwayMax + 1], uint32_t m,
+                                                uint32_t tid) {
+  static_assert(kKwayMax == 8, "three levels");
+  bool bad = false;
+  if (m > 1) bad |= kway_level<U, V, KV, TIES, CHECK, 1>(s_k, s_v, st, m <= 2, tid);
+  if (m > 2) bad |= kway_level<U, V, KV, TIES, CHECK, 2>(s_k, s_v, st, m <= 4, tid);
+  if (m > 4) bad |= kway_level<U, V, KV, TIES, CHECK, 4>(s_k, s_v, st, true, tid);
   if constexpr (CHECK) return __syncthreads_or(bad) != 0;
   return false;
 }
@@ -489,18 +512,22 @@ __global__ __launch_bounds__(kKwayThreads) void kway_merge_kernel(const U* __res
   const uint32_t m = s_part.m;
   const uint64_t out_base = s_part.out;
 
+  uint32_t st[kKwayMax + 1];  // piece starts, uniform
+#pragma unroll
+  for (uint32_t c = 0; c <= kKwayMax; ++c) st[c] = __builtin_amdgcn_readfirstlane(s_start[c]);
+
   kway_stage<U, V, KV, LEX>(sk, sv, s_k, s_v, s_start, s_src, total, tid, xf, vxf);
   __syncthreads();
   if constexpr (LEX) {
     // Keys first; merged again with payloads compared on equal keys only
     // where equal keys of different pieces came out of payload order.
-    if (kway_merge_tree<U, V, KV, false, true>(s_k, s_v, s_start, m, tid)) {
+    if (kway_merge_tree<U, V, KV, false, true>(s_k, s_v, st, m, tid)) {
       kway_stage<U, V, KV, LEX>(sk, sv, s_k, s_v, s_start, s_src, total, tid, xf, vxf);
       __syncthreads();
-      kway_merge_tree<U, V, KV, true, false>(s_k, s_v, s_start, m, tid);
+      kway_merge_tree<U, V, KV, true, false>(s_k, s_v, st, m, tid);
     }
   } else {
-    kway_merge_tree<U, V, KV, false, false>(s_k, s_v, s_start, m, tid);
+    kway_merge_tree<U, V, KV, false, false>(s_k, s_v, st, m, tid);
   }
   for (uint32_t o = tid; o < total; o += kKwayThreads) {
     dk[out_base + o] = from_ordered(s_k[o], xf);
