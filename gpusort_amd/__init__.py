@@ -7,8 +7,10 @@ missing, every call raises ``GpusortError``.
 """
 from __future__ import annotations
 
+import atexit
 import ctypes
 import os
+import weakref
 from pathlib import Path
 
 import numpy as np
@@ -315,6 +317,32 @@ def _dev_ptr(t) -> int:
     return t.data_ptr()
 
 
+# Handles, plans and servers still open when the interpreter exits are
+# closed from an atexit hook, newest first (plans and servers before the
+# handle they use), while the HIP runtime is still whole; closing them later,
+# from __del__ during teardown, would release device and pinned host memory
+# after the runtime (or a profiler hooked into it) has shut down.
+_open_objects: list = []
+
+
+def _track(obj) -> None:
+    if len(_open_objects) >= 64:
+        _open_objects[:] = [r for r in _open_objects if r() is not None]
+    _open_objects.append(weakref.ref(obj))
+
+
+@atexit.register
+def _close_open_objects() -> None:
+    for ref in reversed(_open_objects):
+        obj = ref()
+        if obj is not None:
+            try:
+                obj.close()
+            except Exception:
+                pass
+    _open_objects.clear()
+
+
 class Handle:
     """One library handle: a device, a stream and the HBM arena."""
 
@@ -324,6 +352,7 @@ class Handle:
         _check(self.lib.gs_create(ctypes.byref(h), device))
         self.h = h
         self.device = device
+        _track(self)
         if stream is not None:
             self.set_stream(stream)
 
@@ -459,10 +488,20 @@ class Handle:
         return out.reshape(nruns, spl.size + 2)
 
     def vrank_sort(self, keys, rank_off: np.ndarray, vals=None, algorithm=HYKSORT, k_way=8, samples_per_rank=32,
-                   descending=False, max_splitters=1 << 16, max_buckets=1 << 20):
+                   descending=False, max_splitters=None, max_buckets=None):
         rank_off = np.ascontiguousarray(rank_off, dtype=np.uint64)
         p = rank_off.size - 1
         kcode = torch_dtype_code(keys)
+        # Report room: at most ceil(log2 p) rounds, each with at most
+        # max(k, p) + 1 buckets and max(k, p) splitters per rank (capped at
+        # 2^16 splitters and 2^20 bucket sizes unless given); sized to fit,
+        # since large zeroed host arrays cost more than the sort's tail.
+        fan = max(int(k_way), p, 2)
+        rounds = max(1, (p - 1).bit_length())
+        if max_splitters is None:
+            max_splitters = min(1 << 16, rounds * p * fan + p)
+        if max_buckets is None:
+            max_buckets = min(1 << 20, rounds * p * (fan + 1))
         spl = np.zeros(max_splitters, dtype=_CODE_NP[kcode])
         bsz = np.zeros(max_buckets, dtype=np.uint64)
         fin = np.zeros(p, dtype=np.uint64)
@@ -479,6 +518,10 @@ class Handle:
                 int(descending), ctypes.byref(rep),
             )
         )
+        if rep.n_splitters > max_splitters or rep.n_bucket_sizes > max_buckets:
+            raise GpusortError(1, f"the report holds {rep.n_splitters} splitters and "
+                               f"{rep.n_bucket_sizes} bucket sizes, more than max_splitters={max_splitters} / "
+                               f"max_buckets={max_buckets} (the sort itself completed)")
         return {
             "rounds": rep.rounds,
             "splitters": spl[: rep.n_splitters].copy(),
@@ -546,6 +589,7 @@ class SegPlan:
         _check(self.lib.gs_seg_plan_create(handle.h, kdt, vdt, off.ctypes.data_as(ctypes.POINTER(ctypes.c_uint64)),
                                            off.size - 1, int(descending), ctypes.byref(p)))
         self.p = p.value
+        _track(self)
 
     def run(self, keys, vals=None) -> None:
         _check(self.lib.gs_seg_plan_run(self.p, _dev_ptr(keys), None if vals is None else _dev_ptr(vals)))
@@ -578,6 +622,7 @@ class Server:
         p = ctypes.c_void_p()
         _check(self.lib.gs_server_create(ctypes.byref(p), ctypes.byref(cfg)))
         self.s = p.value
+        _track(self)
         self._live = {}
 
     def submit(self, keys, vals=None, descending: bool = False) -> int:
