@@ -551,6 +551,9 @@ def run_c3(h, g, torch, with_cpu: bool, steps: int = 3):
            "max_rank_error": rep["max_rank_error"],
            "splitter_iterations": rep.get("splitter_iterations"),
            "kernels_ms": {name: round(st["total_ms"] / steps, 3) for name, st in prof.items()}}
+    ksum = sum(st["total_ms"] for st in prof.values()) / steps
+    out["kernel_sum_ms"] = round(ksum, 3)
+    out["wall_over_kernel_sum"] = round(ms / ksum, 4)
     mroof = roofline_of(prof, "kway_merge_kernel")
     if mroof is not None:
         mroof["traffic"] = traffic_from_profile(n, "pmc_kway_merge_vrank_hyksort_2p27_u64_p64.txt")

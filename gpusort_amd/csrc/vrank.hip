@@ -811,7 +811,9 @@ extern "C" gs_status gs_vrank_sort(gs_handle* h, gs_algorithm algo, gs_dtype kdt
     void* host = nullptr;
     e = host_pinned(h, 256, &host);
     if (e == hipSuccess) e = fill_u32(reinterpret_cast<uint32_t*>(ws), 1, 0u, h->stream);
+    if (e == hipSuccess && h->timer()) h->timer()->begin("payload_order_check", n * (uint64_t)P.vb, h->stream);
     if (e == hipSuccess) e = unsorted_flag(P.vb, d_vals, n, P.vx, reinterpret_cast<uint32_t*>(ws), h->stream);
+    if (e == hipSuccess && h->timer()) h->timer()->end(h->stream);
     if (e == hipSuccess) e = hipMemcpyAsync(host, ws, 4, hipMemcpyDeviceToHost, h->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
     if (e != hipSuccess) return from_hip(e, "payload order check");
