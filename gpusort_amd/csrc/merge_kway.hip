@@ -311,8 +311,8 @@ __global__ __launch_bounds__(256) void kway_part_kernel(const KwayDest* __restri
 // piece; every thread's loads are issued before any is used).
 template <typename U, typename V, bool KV, bool LEX>
 __device__ __forceinline__ void kway_stage(const U* __restrict__ sk, const V* __restrict__ sv, U* s_k, V* s_v,
-                                           const uint32_t* s_start, const uint64_t* s_src, uint32_t total,
-                                           uint32_t tid, const Xform<U>& xf, const Xform<V>& vxf) {
+                                           const uint32_t (&st)[kKwayMax + 1], const uint64_t* s_src,
+                                           uint32_t total, uint32_t tid, const Xform<U>& xf, const Xform<V>& vxf) {
   constexpr int VT = kKwayItems;
   U lk[VT];
   V lv[KV ? VT : 1];
@@ -322,7 +322,7 @@ __device__ __forceinline__ void kway_stage(const U* __restrict__ sk, const V* __
     if (e < total) {
       uint32_t r = 0;
 #pragma unroll
-      for (uint32_t c = 1; c < kKwayMax; ++c) r += e >= s_start[c] ? 1u : 0u;
+      for (uint32_t c = 1; c < kKwayMax; ++c) r += e >= st[c] ? 1u : 0u;  // starts: uniform values
       const uint64_t a = s_src[r] + e;
       lk[u] = sk[a];
       if constexpr (KV) lv[u] = sv[a];
@@ -516,13 +516,13 @@ __global__ __launch_bounds__(kKwayThreads) void kway_merge_kernel(const U* __res
 #pragma unroll
   for (uint32_t c = 0; c <= kKwayMax; ++c) st[c] = __builtin_amdgcn_readfirstlane(s_start[c]);
 
-  kway_stage<U, V, KV, LEX>(sk, sv, s_k, s_v, s_start, s_src, total, tid, xf, vxf);
+  kway_stage<U, V, KV, LEX>(sk, sv, s_k, s_v, st, s_src, total, tid, xf, vxf);
   __syncthreads();
   if constexpr (LEX) {
     // Keys first; merged again with payloads compared on equal keys only
     // where equal keys of different pieces came out of payload order.
     if (kway_merge_tree<U, V, KV, false, true>(s_k, s_v, st, m, tid)) {
-      kway_stage<U, V, KV, LEX>(sk, sv, s_k, s_v, s_start, s_src, total, tid, xf, vxf);
+      kway_stage<U, V, KV, LEX>(sk, sv, s_k, s_v, st, s_src, total, tid, xf, vxf);
       __syncthreads();
       kway_merge_tree<U, V, KV, true, false>(s_k, s_v, st, m, tid);
     }
