@@ -492,16 +492,11 @@ class Handle:
         rank_off = np.ascontiguousarray(rank_off, dtype=np.uint64)
         p = rank_off.size - 1
         kcode = torch_dtype_code(keys)
-        # Report room: at most ceil(log2 p) rounds, each with at most
-        # max(k, p) + 1 buckets and max(k, p) splitters per rank (capped at
-        # 2^16 splitters and 2^20 bucket sizes unless given); sized to fit,
-        # since large zeroed host arrays cost more than the sort's tail.
-        fan = max(int(k_way), p, 2)
-        rounds = max(1, (p - 1).bit_length())
+        caps = report_capacity(p, k_way)
         if max_splitters is None:
-            max_splitters = min(1 << 16, rounds * p * fan + p)
+            max_splitters = caps[0]
         if max_buckets is None:
-            max_buckets = min(1 << 20, rounds * p * (fan + 1))
+            max_buckets = caps[1]
         spl = np.zeros(max_splitters, dtype=_CODE_NP[kcode])
         bsz = np.zeros(max_buckets, dtype=np.uint64)
         fin = np.zeros(p, dtype=np.uint64)
@@ -684,6 +679,17 @@ class Server:
             self.close()
         except Exception:
             pass
+
+
+def report_capacity(p: int, k_way: int) -> tuple[int, int]:
+    """Room for a virtual-rank report: (splitters, bucket sizes).  Every
+    algorithm runs at most ceil(log2 p) rounds, each with at most
+    max(k, p) + 1 buckets and max(k, p) splitters per rank; capped at 2^16
+    splitters and 2^20 bucket sizes.  Sized to fit rather than to the caps,
+    since large zeroed host arrays cost more than the sort's tail."""
+    fan = max(int(k_way), int(p), 2)
+    rounds = max(1, (int(p) - 1).bit_length())
+    return min(1 << 16, rounds * p * fan + p), min(1 << 20, rounds * p * (fan + 1))
 
 
 def rebalance_sizes(sizes) -> np.ndarray:
