@@ -6,7 +6,8 @@ inputs bench.py measures):
   payload maps input to output, ties in input order);
 - 64 virtual ranks x 2^21 u64 keys + u32 payloads, k = 8 HykSort with 32
   samples per rank, against the oracle's re-run of the same rounds (keys,
-  payloads, splitters, bucket sizes, final rank sizes);
+  payloads, splitters, bucket sizes, final rank sizes); the same shape at
+  2^19 keys per rank with 1000 distinct keys and random payloads;
 - 100,000 arrays of lognormal lengths (median 2048, sigma 2, at most 2^22) in
   one batched call, by per-array properties;
 - served requests of log-uniform sizes up to 2^28 and mixed types, each
@@ -81,6 +82,32 @@ def test_vrank_hyksort_64_ranks_2p21_u64(handle):
     assert np.array_equal(rep["final_rank_sizes"].astype(np.int64), want["final_rank_sizes"]), "final rank sizes"
     assert rep["splitter_iterations"] == want["splitter_iterations"]
     assert rep["max_rank_error"] <= 0.01 / 7
+
+
+def test_vrank_64_ranks_tied_keys_random_payloads(handle):
+    """The 64-rank HykSort shape at 2^19 keys per rank, with 1000 distinct
+    u64 keys and random u32 payloads: every merge partition meets equal keys
+    whose payloads decide the order (the merge's payload-order check and
+    re-merge), checked against the oracle's re-run and the (key, payload)
+    order of the input."""
+    p, n = 64, 1 << 25
+    off = np.arange(p + 1, dtype=np.uint64) * np.uint64(n // p)
+    pool = datagen.uniform_u64(11, 1000)
+    hk = pool[datagen.draws(12, n) % np.uint64(1000)]
+    hv = (datagen.draws(13, n) >> np.uint64(32)).astype(np.uint32)
+    keys = torch.from_numpy(hk.view(np.int64)).cuda().view(torch.uint64)
+    vals = torch.from_numpy(hv.view(np.int32)).cuda().view(torch.uint32)
+    say(f"vrank HykSort tied keys p={p} n={n} k=8 s=32")
+    rep = handle.vrank_sort(keys, off, vals, algorithm=g.HYKSORT, k_way=8, samples_per_rank=32)
+    wk, wv, want = oracle.vrank(hk, off.astype(np.int64), hv, algorithm="hyksort", k_way=8, samples_per_rank=32)
+    gk, gv = keys.cpu().numpy(), vals.cpu().numpy()
+    assert np.array_equal(gk, wk), "keys"
+    assert np.array_equal(gv, wv), "payloads"
+    assert np.array_equal(rep["splitters"], want["splitters"]), "splitters"
+    assert np.array_equal(rep["bucket_sizes"].astype(np.int64), want["bucket_sizes"]), "bucket sizes"
+    assert np.array_equal(rep["final_rank_sizes"].astype(np.int64), want["final_rank_sizes"]), "final rank sizes"
+    order = np.lexsort((hv, hk))
+    assert np.array_equal(gk, hk[order]) and np.array_equal(gv, hv[order]), "(key, payload) order"
 
 
 def test_batched_100k_lognormal_arrays(handle):
