@@ -186,7 +186,7 @@ extern "C" gs_status gs_sort_segments(gs_handle* h, gs_dtype kdt, void* d_keys, 
   if (!dtype_ok(kdt)) return fail(GS_INVALID_ARG, "unsupported key type %d", (int)kdt);
   if (vdt != GS_NONE && !dtype_ok(vdt)) return fail(GS_INVALID_ARG, "unsupported value type %d", (int)vdt);
   if (nseg == 0) return GS_OK;
-  if (!d_offsets || !d_keys || (vdt != GS_NONE && !d_vals)) return fail(GS_INVALID_ARG, "null pointer");
+  if (!d_offsets) return fail(GS_INVALID_ARG, "null pointer");
   DeviceScope ds(h->device);
   const int kb = dtype_bytes(kdt), vb = val_bytes_of(vdt);
   std::vector<uint64_t> off(nseg + 1);
@@ -195,6 +195,8 @@ extern "C" gs_status gs_sort_segments(gs_handle* h, gs_dtype kdt, void* d_keys, 
   if (e != hipSuccess) return from_hip(e, "read segment offsets");
   for (uint32_t s = 0; s < nseg; ++s)
     if (off[s + 1] < off[s]) return fail(GS_INVALID_ARG, "segment offsets decrease at segment %u", s);
+  if (off[nseg] == off[0]) return GS_OK;  // every segment empty: nothing is read (null pointers allowed)
+  if (!d_keys || (vdt != GS_NONE && !d_vals)) return fail(GS_INVALID_ARG, "null pointer");
   std::vector<unsigned char> meta;
   std::vector<uint32_t> huge;
   gs_status st = sort_segments_async(h, kdt, d_keys, vdt, d_vals, off.data(), nseg, descending, &meta, &huge);

@@ -143,7 +143,8 @@ gs_status gs_sort_pairs_host(gs_handle* h, gs_dtype kdt, void* h_keys, gs_dtype 
 /* Batched segmented sort: segment s is [d_offsets[s], d_offsets[s+1]) of the
  * device arrays (offsets in device memory, nseg+1 entries, non-decreasing).
  * Every segment is sorted independently (stable with payload) in one call.
- * d_vals may be NULL with vdt GS_NONE.  No reference counterpart as one call:
+ * d_vals may be NULL with vdt GS_NONE; d_keys and d_vals may be NULL when
+ * every segment is empty.  No reference counterpart as one call:
  * it batches many CudaUtils::Sort / SortByKey calls (src/cuda_utils.cu:97-243). */
 gs_status gs_sort_segments(gs_handle* h, gs_dtype kdt, void* d_keys, gs_dtype vdt, void* d_vals,
                            const uint64_t* d_offsets, uint32_t nseg, int descending);
