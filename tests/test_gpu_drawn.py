@@ -1,12 +1,14 @@
 """Seeded drawn configurations beyond the hand-picked ones: single-array
 sorts (key type, payload type, direction, size, distribution), batches of
-segments and multisplits, every one against the oracle bit for bit."""
+segments, multisplits, 2-way merges, gathers, host-pointer sorts and segment
+plans with graph replays, every one against the oracle bit for bit."""
 from __future__ import annotations
 
 import numpy as np
 import pytest
 import torch
 
+import gpusort_amd as g
 import oracle
 from gpusort_amd import datagen
 from tests.gpu_util import assert_bits_equal, dev, host, say
@@ -112,3 +114,98 @@ def _multisplit_cases():
 def test_drawn_multisplits(handle, case):
     kdt, vdt, desc, upper, n, nspl, seed, distinct = case
     multisplit_run(handle, kdt, vdt, n, nspl, upper, desc, seed=seed, distinct=distinct)
+
+
+MERGE_SIZES = [0, 1, 5, 1000, 2047, 2048, 2049, 70001, 1 << 20]
+
+
+def _merge_cases():
+    return [(KEY_TYPES[x[0] % 6], bool(x[1] & 1), MERGE_SIZES[x[2] % 9], MERGE_SIZES[x[3] % 9], bool(x[4] & 1),
+             x[5] % 9999) for x in _draws(7171, 8)]
+
+
+@pytest.mark.parametrize("case", _merge_cases(), ids=lambda c: f"{_name(c[0])}-{'desc' if c[1] else 'asc'}-"
+                         f"{c[2]}x{c[3]}{'-ties' if c[4] else ''}")
+def test_drawn_merges(handle, case):
+    kdt, desc, na, nb, ties, seed = case
+    a = make_keys(kdt, na, seed=seed)
+    b = make_keys(kdt, nb, seed=seed + 1)
+    if ties and na:
+        a = a[datagen.draws(seed + 2, na) % np.uint64(max(1, na // 20))]
+        if nb:
+            b = a[datagen.draws(seed + 3, nb) % np.uint64(na)]  # b's keys all occur in a
+    a = oracle.std_sort(np.ascontiguousarray(a), desc)
+    b = oracle.std_sort(np.ascontiguousarray(b), desc)
+    say(f"drawn merge {case}")
+    ok, ov = dev(np.zeros(na + nb, dtype=kdt)), dev(np.zeros(na + nb, dtype=np.uint32))
+    handle.merge(dev(a), dev(b), ok, dev(np.arange(na, dtype=np.uint32)), dev(np.arange(na, na + nb, dtype=np.uint32)),
+                 ov, descending=desc)
+    assert_bits_equal(host(ok), oracle.merge(a, b, desc), "keys")
+    cat = np.concatenate([a, b])
+    order = np.argsort(-cat if np.dtype(kdt).kind == "f" and desc else (~cat if desc else cat), kind="stable")
+    assert_bits_equal(host(ov), order.astype(np.uint32), "ties from the first run")
+
+
+@pytest.mark.parametrize("case", [(np.uint16, 70001, 1000), (np.uint32, 1 << 20, 1 << 20), (np.uint64, 1000, 5),
+                                  (np.uint32, 0, 7), (np.uint64, 1, 1), (np.uint16, 2049, 1 << 20)],
+                         ids=lambda c: f"{_name(c[0])}-n{c[1]}-src{c[2]}")
+def test_drawn_gathers(handle, case):
+    et, n, nsrc = case
+    src = (datagen.draws(nsrc, nsrc) >> np.uint64(8)).astype(et)
+    idx = (datagen.draws(n + 3, n) % np.uint64(nsrc)).astype(np.int64)  # with repeats
+    say(f"drawn gather {case}")
+    dst = dev(np.zeros(n, dtype=et))
+    handle.gather(dev(src), dev(idx), dst)
+    assert np.array_equal(host(dst), src[idx])
+
+
+@pytest.mark.parametrize("case", [(KEY_TYPES[x[0] % 6], bool(x[1] & 1), MERGE_SIZES[x[2] % 9], x[3] % 999)
+                                  for x in _draws(8181, 6)],
+                         ids=lambda c: f"{_name(c[0])}-{'desc' if c[1] else 'asc'}-n{c[2]}")
+def test_drawn_host_pointer_sorts(handle, case):
+    kdt, desc, n, seed = case
+    keys = np.ascontiguousarray(make_keys(kdt, n, seed=seed))
+    say(f"drawn host sort {case}")
+    k = keys.copy()
+    handle.sort_keys_host(k, descending=desc)
+    assert_bits_equal(k, oracle.merge_sort(keys, descending=desc), "keys")
+    vals = np.arange(n, dtype=np.uint32)
+    k, v = keys.copy(), vals.copy()
+    handle.sort_pairs_host(k, v, descending=desc)
+    wk, wv = oracle.stable_sort_pairs(keys, vals, descending=desc)
+    assert_bits_equal(k, wk, "pair keys")
+    assert np.array_equal(v, wv)
+
+
+@pytest.mark.parametrize("case", [(np.uint32, True, False, 500, 3000), (np.uint64, False, True, 10, 100000),
+                                  (np.float32, True, True, 1, 0), (np.uint32, False, False, 500, 3)],
+                         ids=lambda c: f"{_name(c[0])}-{'kv' if c[1] else 'keys'}-{'desc' if c[2] else 'asc'}"
+                         f"-s{c[3]}-max{c[4]}")
+def test_drawn_segment_plans(handle, case):
+    """A plan run once, then replayed as a graph twice with eager sorts in
+    between; segments include empty ones (all empty in one case)."""
+    kdt, kv, desc, nseg, hi = case
+    lengths = (datagen.draws(nseg + hi, nseg) % np.uint64(hi + 1)).astype(np.int64)
+    off = np.concatenate([[0], np.cumsum(lengths)]).astype(np.uint64)
+    n = int(off[-1])
+    keys = np.ascontiguousarray(make_keys(kdt, n, seed=nseg))
+    vals = np.arange(n, dtype=np.uint32)
+    if kv:
+        wk, wv = oracle.seg_stable_sort_pairs(keys, vals, off.astype(np.int64), descending=desc)
+    else:
+        wk, wv = oracle.seg_sort(keys, off.astype(np.int64), descending=desc), None
+    code = {np.dtype(np.uint32): g.U32, np.dtype(np.uint64): g.U64, np.dtype(np.float32): g.F32}[np.dtype(kdt)]
+    plan = handle.seg_plan(off, code, g.U32 if kv else g.NONE, descending=desc)
+    say(f"drawn plan {case} n={n}")
+    for rep in range(3):
+        tk, tv = dev(keys), dev(vals) if kv else None
+        if rep == 0:
+            plan.run(tk, tv)
+        else:
+            plan.launch_graph(tk, tv)
+        torch.cuda.synchronize()
+        assert_bits_equal(host(tk), wk, f"keys, replay {rep}")
+        if kv:
+            assert np.array_equal(host(tv), wv), f"payloads, replay {rep}"
+        handle.sort_keys(dev(make_keys(np.uint32, 5000, seed=rep)))  # eager work between replays
+    plan.close()
