@@ -345,14 +345,18 @@ def run_served(h, g, torch, n_requests: int, seed: int = 5, loads=(0.5, 0.9, 1.5
     return out
 
 
-def run_multisplit(h, g, torch, n: int, nspl: int = 63, steps: int = 5):
+def run_multisplit(h, g, torch, n: int, nspl: int = 63, steps: int = 10, warmup: int = 10):
     """Stable multisplit of 2^28 uniform u32 keys + u32 payloads into 64
-    buckets (splitters = evenly spaced sorted sample of the keys)."""
+    buckets (splitters = evenly spaced sorted sample of the keys).  The
+    warm-up calls bring the GPU to its steady clocks after the host-only
+    work before this leg (the first calls after an idle spell run up to 15 %
+    slower)."""
     keys, vals = make_input(h, g, torch, "uniform", n, SEED)
     sample = keys[:: max(1, n // 4096)].cpu().numpy()
     spl = np.sort(sample)[np.linspace(0, sample.size - 1, nspl + 2).astype(np.int64)[1:-1]]
     ok, ov = torch.empty_like(keys), torch.empty_like(vals)
-    h.multisplit(keys, spl, ok, vals, ov)
+    for _ in range(warmup):
+        h.multisplit(keys, spl, ok, vals, ov)
     torch.cuda.synchronize()
     h.profile_begin()
     t0 = time.perf_counter()

@@ -377,10 +377,13 @@ __device__ inline void load_split_tab(uint16_t* ent, U* img, const SplitTab<U>* 
   for (uint32_t j = tid; j < kRadix; j += nthreads) img[j] = T->img[j];
 }
 
-// Buckets of N images at once: every entry is read first, then the entries
-// that hold splitters are narrowed together, one branch-free step per key
-// per round (an idle key re-reads a valid image and keeps its range), so
-// the LDS reads of all N keys are in flight together.
+// Buckets of N images at once: every entry is read first; then every key
+// takes one branch-free step against the splitter at the low end of its
+// range (an entry holding one splitter, the common case, is decided there,
+// with no loop); entries holding several splitters are then narrowed
+// together, one branch-free step per key per round (an idle key re-reads a
+// valid image and keeps its range), so the LDS reads of all N keys are in
+// flight together.
 template <typename U, int N>
 __device__ inline void tab_buckets(const uint16_t* ent, const U* img, const U (&x)[N], U base, uint32_t shift,
                                    int upper, uint32_t (&b)[N]) {
@@ -393,7 +396,14 @@ __device__ inline void tab_buckets(const uint16_t* ent, const U* img, const U (&
   }
   bool more = false;
 #pragma unroll
-  for (int i = 0; i < N; ++i) more |= lo[i] < hi[i];
+  for (int i = 0; i < N; ++i) {
+    const U t = img[lo[i]];  // lo <= nspl <= 255: a valid image even when idle
+    const bool act = lo[i] < hi[i];
+    const bool right = upper ? (t < x[i]) : (t <= x[i]);
+    hi[i] = (act && !right) ? lo[i] : hi[i];  // every later splitter is above x too
+    lo[i] = (act && right) ? lo[i] + 1 : lo[i];
+    more |= lo[i] < hi[i];
+  }
   while (more) {
     more = false;
 #pragma unroll
@@ -1409,12 +1419,16 @@ hipError_t radix_sort_segments(int kb, int vb, void* keys, void* vals, const Seg
 // perfect splitter tree in LDS.  Output goes to separate buffers, bucket after bucket,
 // keys of a bucket in input order.
 
-// Bucket counts of a multisplit in one read of the keys: buckets from the
-// bucket table in LDS, counted in kCountCopies copies of the 256 counters,
-// lane l using copy l % 32 (its bank), so the counts add no conflict across
-// lanes.
+// Bucket counts of a multisplit in one read of the keys, counted in
+// kCountCopies copies of the 256 counters, lane l using copy l % 32 (its
+// bank), so the counts add no conflict across lanes.  Each workgroup first
+// refines the bucket table 4x in LDS (kFineTab entries over the same image
+// range, each searched only within the range its coarse entry gives), so
+// that only keys in the few entries that hold a splitter search the
+// splitter images; keys below the smallest splitter are bucket 0 at once.
 constexpr int kSplitHistThreads = 1024;
 constexpr uint32_t kCountCopies = 32;
+constexpr uint32_t kFineTab = 4 * kSplitTab;
 
 template <typename U>
 __global__ __launch_bounds__(kSplitHistThreads) void split_hist_kernel(
@@ -1422,6 +1436,7 @@ __global__ __launch_bounds__(kSplitHistThreads) void split_hist_kernel(
     unsigned long long* __restrict__ g_hist, uint4* __restrict__ zero_ptr, uint64_t zero_vec4) {
   __shared__ __attribute__((aligned(16))) uint32_t s_cnt[kRadix * kCountCopies];  // [bucket][copy]
   __shared__ __attribute__((aligned(16))) uint16_t s_ent[kSplitTab];
+  __shared__ __attribute__((aligned(16))) uint16_t s_fine[kFineTab];
   __shared__ U s_img[kRadix];
   const uint32_t tid = threadIdx.x;
   const uint32_t copy = tid & (kCountCopies - 1);
@@ -1430,15 +1445,49 @@ __global__ __launch_bounds__(kSplitHistThreads) void split_hist_kernel(
   load_split_tab<U>(s_ent, s_img, T, tid, kSplitHistThreads);
   const U tab_base = T->base;
   const uint32_t tab_shift = T->shift;
+  const uint32_t nspl = T->nspl;
   const uint64_t gtid = (uint64_t)blockIdx.x * kSplitHistThreads + tid;
   const uint64_t gstride = (uint64_t)gridDim.x * kSplitHistThreads;
   for (uint64_t i = gtid; i < zero_vec4; i += gstride) zero_ptr[i] = make_uint4(0, 0, 0, 0);
   __syncthreads();
+  // Fine entry f covers images [base + (f << fshift), base + ((f + 1) << fshift) - 1]
+  // (the last reachable entry up to the largest image); its splitter counts
+  // lie within those of the coarse entries holding its two ends.
+  const uint32_t fshift = tab_shift >= 2 ? tab_shift - 2 : 0;
+  const U flast = (U)(U(~U(0)) - tab_base) >> fshift;  // entry of the largest image, before the clamp
+  for (uint32_t f = tid; f < kFineTab; f += kSplitHistThreads) {
+    uint32_t lo, hi;
+    if (f != 0 && (U)f > flast) {
+      lo = hi = nspl;  // no image maps here
+    } else {
+      const U first = (U)(tab_base + ((U)f << fshift));
+      const bool to_top = f == kFineTab - 1 || (U)f == flast;
+      const U end = to_top ? U(~U(0)) : (U)(tab_base + ((U)(f + 1) << fshift) - 1);
+      const uint32_t ef = s_ent[tab_entry_of<U>(first, tab_base, tab_shift)];
+      const uint32_t ee = s_ent[tab_entry_of<U>(end, tab_base, tab_shift)];
+      lo = count_splitters<U>(s_img, ef & 0xFFu, ef >> 8, first, upper);
+      hi = count_splitters<U>(s_img, lo, ee >> 8, end, upper);
+    }
+    s_fine[f] = (uint16_t)(lo | (hi << 8));
+  }
+  __syncthreads();
   auto count = [&](U key) {
-    U x[1] = {to_ordered(key, xf)};
-    uint32_t b[1];
-    tab_buckets<U, 1>(s_ent, s_img, x, tab_base, tab_shift, upper, b);
-    atomicAdd(&s_cnt[b[0] * kCountCopies + copy], 1u);
+    const U x = to_ordered(key, xf);
+    uint32_t e = 0;  // below the smallest splitter: bucket 0
+    if (x >= tab_base) {
+      const U q = (U)(x - tab_base) >> fshift;
+      e = s_fine[q < (U)kFineTab ? (uint32_t)q : kFineTab - 1];
+    }
+    uint32_t lo = e & 0xFFu, hi = e >> 8;
+    while (lo < hi) {  // only keys in an entry that holds splitters
+      const uint32_t mid = (lo + hi) >> 1;
+      const U t = s_img[mid];
+      if (upper ? (t < x) : (t <= x))
+        lo = mid + 1;
+      else
+        hi = mid;
+    }
+    atomicAdd(&s_cnt[lo * kCountCopies + copy], 1u);
   };
   // 16-byte loads, two in flight per thread.
   using VT = typename Vec16<U>::T;
