@@ -1414,10 +1414,10 @@ hipError_t radix_sort_segments(int kb, int vb, void* keys, void* vals, const Seg
 // ------------------------------------------------------------------ multisplit
 //
 // Stable multisplit of unsorted keys (and payloads) into nspl + 1 <= 256
-// buckets: one counting read (split_hist_kernel), the scan, and one onesweep
-// pass whose digit is the key's bucket, found by log2(nspl + 1) steps down a
-// perfect splitter tree in LDS.  Output goes to separate buffers, bucket after bucket,
-// keys of a bucket in input order.
+// buckets: the bucket table (split_tab_kernel), one counting read
+// (split_hist_kernel), the scan, and one onesweep pass whose digit is the
+// key's bucket, read from the table in LDS.  Output goes to separate
+// buffers, bucket after bucket, keys of a bucket in input order.
 
 // Bucket counts of a multisplit in one read of the keys, counted in
 // kCountCopies copies of the 256 counters, lane l using copy l % 32 (its
