@@ -177,6 +177,7 @@ _SIGS = {
     "gs_destroy": ([ctypes.c_void_p], ctypes.c_int),
     "gs_set_stream": ([ctypes.c_void_p, ctypes.c_void_p], ctypes.c_int),
     "gs_set_tile_order": ([ctypes.c_void_p, ctypes.c_int], ctypes.c_int),
+    "gs_set_host_sort_budget": ([ctypes.c_void_p, ctypes.c_uint64], ctypes.c_int),
     "gs_synchronize": ([ctypes.c_void_p], ctypes.c_int),
     "gs_arena_bytes": ([ctypes.c_void_p, ctypes.POINTER(ctypes.c_uint64)], ctypes.c_int),
     "gs_release_arena": ([ctypes.c_void_p], ctypes.c_int),
@@ -375,6 +376,12 @@ class Handle:
         """TILES_AUTO (grouped tiles on the null stream, arrival order
         elsewhere), TILES_ARRIVAL or TILES_GROUPED (gs_set_tile_order)."""
         _check(self.lib.gs_set_tile_order(self.h, int(order)))
+
+    def set_host_sort_budget(self, nbytes: int) -> None:
+        """Device bytes one host-memory sort may stage (gs_set_host_sort_budget);
+        0 means the device's free memory.  Larger inputs are sorted in chunks
+        and merged through the device."""
+        _check(self.lib.gs_set_host_sort_budget(self.h, int(nbytes)))
 
     def use_torch_stream(self) -> None:
         import torch

@@ -15,6 +15,7 @@
 
 #include "../../include/gpusort_hip.h"
 #include "gs_capi_impl.h"
+#include "gs_device.h"
 #include "gs_internal.h"
 
 namespace gs {
@@ -153,6 +154,190 @@ gs_status sort_device(gs_handle* h, gs_dtype kdt, void* d_keys, gs_dtype vdt, vo
                   "radix sort");
 }
 
+// ------------------------------------------------------------ host chunks
+//
+// Host-memory sorts of inputs larger than the device can stage, as the
+// reference's CudaUtils::Sort / SortByKey do (src/cuda_utils.cu:97-167,
+// :169-243): chunks of budget / (3 * element bytes) elements
+// (SplitChunks, :508-520, with the element = key + payload), each sorted on
+// the device, then a binary tree of stable 2-way merges of neighbouring runs
+// (ParallelMergeArrays, :323-371), ping-ponging between the caller's arrays
+// and a host buffer so that the last level lands in the caller's arrays.
+// Each 2-way merge streams through the device in output blocks of at most a
+// chunk: the block's split of the two runs is a merge-path search over the
+// ordered key images on the host, the two input slices go in, the device
+// merge (ties from the earlier run) writes the block, and it comes back.
+
+// Elements one device-staged piece may hold; 0 when not even one fits.
+static uint64_t host_chunk_elems(gs_handle* h, int kb, int vb) {
+  uint64_t budget = h->host_budget;
+  if (budget == 0) {
+    size_t free_b = 0, total_b = 0;
+    if (hipMemGetInfo(&free_b, &total_b) != hipSuccess) return 0;
+    budget = free_b;
+  }
+  return budget / (3ull * (uint64_t)(kb + vb));
+}
+
+// Number of elements of a (na) among the first d of the stable merge of a
+// and b (ties from a), by a binary search over the ordered images.
+template <typename U>
+static uint64_t host_corank(const U* a, uint64_t na, const U* b, uint64_t nb, uint64_t d, const Xform<U>& xf) {
+  uint64_t lo = d > nb ? d - nb : 0, hi = d < na ? d : na;
+  while (lo < hi) {
+    const uint64_t mid = lo + (hi - lo) / 2;
+    if (to_ordered(a[mid], xf) <= to_ordered(b[d - mid - 1], xf))
+      lo = mid + 1;
+    else
+      hi = mid;
+  }
+  return lo;
+}
+
+static uint64_t corank_of(int kb, const void* a, uint64_t na, const void* b, uint64_t nb, uint64_t d,
+                          const XformParams& xp) {
+  if (kb == 4) {
+    const Xform<uint32_t> x{(uint32_t)xp.float_mask, (uint32_t)xp.sign_flip, (uint32_t)xp.desc_mask};
+    return host_corank<uint32_t>(static_cast<const uint32_t*>(a), na, static_cast<const uint32_t*>(b), nb, d, x);
+  }
+  const Xform<uint64_t> x{xp.float_mask, xp.sign_flip, xp.desc_mask};
+  return host_corank<uint64_t>(static_cast<const uint64_t*>(a), na, static_cast<const uint64_t*>(b), nb, d, x);
+}
+
+// Stable merge of host runs a = keys[a0, a0 + na), b = the na.. that follow
+// it in the same arrays (src), into dst at the same offsets, through the
+// device in blocks of at most `block` elements.
+static gs_status host_merge_runs(gs_handle* h, gs_dtype kdt, gs_dtype vdt, const unsigned char* src_k,
+                                 const unsigned char* src_v, unsigned char* dst_k, unsigned char* dst_v, uint64_t a0,
+                                 uint64_t na, uint64_t nb, uint64_t block, int descending) {
+  const int kb = dtype_bytes(kdt), vb = val_bytes_of(vdt);
+  const XformParams xp = make_xform((int)kdt, descending != 0);
+  const unsigned char* ak = src_k + a0 * kb;
+  const unsigned char* bk = ak + na * kb;
+  const unsigned char* av = vb ? src_v + a0 * vb : nullptr;
+  const unsigned char* bv = vb ? av + na * vb : nullptr;
+  void *din_k = nullptr, *din_v = nullptr, *dout_k = nullptr, *dout_v = nullptr;
+  if (h->arena.get(Arena::kStageKeys, block * kb, h->stream, &din_k) != hipSuccess ||
+      h->arena.get(Arena::kPipeKeys, block * kb, h->stream, &dout_k) != hipSuccess ||
+      (vb && (h->arena.get(Arena::kStageVals, block * vb, h->stream, &din_v) != hipSuccess ||
+              h->arena.get(Arena::kPipeVals, block * vb, h->stream, &dout_v) != hipSuccess)))
+    return fail(GS_OUT_OF_MEMORY, "Couldn't allocate device vector");
+  const uint64_t total = na + nb;
+  uint64_t i0 = 0;
+  for (uint64_t o = 0; o < total; o += block) {
+    const uint64_t o2 = std::min(total, o + block);
+    const uint64_t i1 = corank_of(kb, ak, na, bk, nb, o2, xp);
+    const uint64_t j0 = o - i0, j1 = o2 - i1;
+    const uint64_t ca = i1 - i0, cb = j1 - j0;
+    unsigned char* ik = static_cast<unsigned char*>(din_k);
+    unsigned char* iv = static_cast<unsigned char*>(din_v);
+    hipError_t e = hipSuccess;
+    if (ca) e = hipMemcpyAsync(ik, ak + i0 * kb, ca * kb, hipMemcpyHostToDevice, h->stream);
+    if (e == hipSuccess && cb) e = hipMemcpyAsync(ik + ca * kb, bk + j0 * kb, cb * kb, hipMemcpyHostToDevice, h->stream);
+    if (vb) {
+      if (e == hipSuccess && ca) e = hipMemcpyAsync(iv, av + i0 * vb, ca * vb, hipMemcpyHostToDevice, h->stream);
+      if (e == hipSuccess && cb)
+        e = hipMemcpyAsync(iv + ca * vb, bv + j0 * vb, cb * vb, hipMemcpyHostToDevice, h->stream);
+    }
+    if (e != hipSuccess) return from_hip(e, "copy merge block to device");
+    const gs_status st = gs_merge(h, kdt, ik, vb ? iv : nullptr, ca, ik + ca * kb, vb ? iv + ca * vb : nullptr, cb,
+                                  vb ? vdt : GS_NONE, dout_k, vb ? dout_v : nullptr, descending);
+    if (st != GS_OK) return st;
+    e = hipMemcpyAsync(dst_k + (a0 + o) * kb, dout_k, (o2 - o) * kb, hipMemcpyDeviceToHost, h->stream);
+    if (e == hipSuccess && vb)
+      e = hipMemcpyAsync(dst_v + (a0 + o) * vb, dout_v, (o2 - o) * vb, hipMemcpyDeviceToHost, h->stream);
+    if (e != hipSuccess) return from_hip(e, "copy merge block to host");
+    i0 = i1;
+  }
+  return from_hip(hipStreamSynchronize(h->stream), "synchronize");
+}
+
+// Packs sorted host keys and values into pair records through the device,
+// `block` elements at a time.
+static gs_status host_pack_pairs(gs_handle* h, int kb, const void* h_keys, int vb, const void* h_vals, uint64_t n,
+                                 void* h_pairs_out, uint32_t stride, uint32_t val_offset, uint64_t block) {
+  void *dk = nullptr, *dv = nullptr, *dp = nullptr;
+  if (h->arena.get(Arena::kStageKeys, block * kb, h->stream, &dk) != hipSuccess ||
+      h->arena.get(Arena::kStageVals, block * vb, h->stream, &dv) != hipSuccess ||
+      h->arena.get(Arena::kStagePairs, block * stride, h->stream, &dp) != hipSuccess)
+    return fail(GS_OUT_OF_MEMORY, "Couldn't allocate device vector");
+  const unsigned char* hk = static_cast<const unsigned char*>(h_keys);
+  const unsigned char* hv = static_cast<const unsigned char*>(h_vals);
+  unsigned char* hp = static_cast<unsigned char*>(h_pairs_out);
+  for (uint64_t s = 0; s < n; s += block) {
+    const uint64_t m = std::min(block, n - s);
+    hipError_t e = hipMemcpyAsync(dk, hk + s * kb, m * kb, hipMemcpyHostToDevice, h->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(dv, hv + s * vb, m * vb, hipMemcpyHostToDevice, h->stream);
+    if (e == hipSuccess) e = pack_pairs(kb, dk, vb, dv, dp, m, stride, val_offset, h->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(hp + s * stride, dp, m * stride, hipMemcpyDeviceToHost, h->stream);
+    if (e != hipSuccess) return from_hip(e, "pack pairs");
+  }
+  return from_hip(hipStreamSynchronize(h->stream), "synchronize");
+}
+
+// Chunked sort of host keys (and payloads) in place; `chunk` < n.
+static gs_status host_sort_chunked(gs_handle* h, gs_dtype kdt, void* h_keys, gs_dtype vdt, void* h_vals, uint64_t n,
+                                   int descending, uint64_t chunk) {
+  const int kb = dtype_bytes(kdt), vb = val_bytes_of(vdt);
+  const uint64_t nchunks = (n + chunk - 1) / chunk;
+  int height = 0;
+  while ((1ull << height) < nchunks) ++height;
+  std::vector<unsigned char> tmp_k, tmp_v;
+  try {
+    tmp_k.resize(n * kb);
+    if (vb) tmp_v.resize(n * vb);
+  } catch (...) {
+    return fail(GS_OUT_OF_MEMORY, "host allocation failed");
+  }
+  unsigned char* user_k = static_cast<unsigned char*>(h_keys);
+  unsigned char* user_v = static_cast<unsigned char*>(h_vals);
+  // The sorted chunks go where an even number of merge levels leaves the
+  // result in the caller's arrays.
+  unsigned char* cur_k = (height % 2) ? tmp_k.data() : user_k;
+  unsigned char* cur_v = (height % 2) ? tmp_v.data() : user_v;
+  unsigned char* oth_k = (height % 2) ? user_k : tmp_k.data();
+  unsigned char* oth_v = (height % 2) ? user_v : tmp_v.data();
+  void *dk = nullptr, *dv = nullptr;
+  if (h->arena.get(Arena::kStageKeys, chunk * kb, h->stream, &dk) != hipSuccess ||
+      (vb && h->arena.get(Arena::kStageVals, chunk * vb, h->stream, &dv) != hipSuccess))
+    return fail(GS_OUT_OF_MEMORY, "Couldn't allocate device vector");
+  for (uint64_t s = 0; s < n; s += chunk) {
+    const uint64_t m = std::min(chunk, n - s);
+    hipError_t e = hipMemcpyAsync(dk, user_k + s * kb, m * kb, hipMemcpyHostToDevice, h->stream);
+    if (e == hipSuccess && vb) e = hipMemcpyAsync(dv, user_v + s * vb, m * vb, hipMemcpyHostToDevice, h->stream);
+    if (e != hipSuccess) return from_hip(e, "copy chunk to device");
+    const gs_status st = sort_device(h, kdt, dk, vdt, dv, m, descending);
+    if (st != GS_OK) return st;
+    e = hipMemcpyAsync(cur_k + s * kb, dk, m * kb, hipMemcpyDeviceToHost, h->stream);
+    if (e == hipSuccess && vb) e = hipMemcpyAsync(cur_v + s * vb, dv, m * vb, hipMemcpyDeviceToHost, h->stream);
+    if (e != hipSuccess) return from_hip(e, "copy chunk to host");
+  }
+  hipError_t e = hipStreamSynchronize(h->stream);
+  if (e != hipSuccess) return from_hip(e, "synchronize");
+  // Merge levels: runs of `run` elements pair up; a run without a partner
+  // is copied as it is.
+  for (uint64_t run = chunk; run < n; run *= 2) {
+    for (uint64_t a0 = 0; a0 < n; a0 += 2 * run) {
+      const uint64_t na = std::min(run, n - a0);
+      const uint64_t nb = std::min(run, n - a0 - na);
+      if (nb == 0) {
+        std::memcpy(oth_k + a0 * kb, cur_k + a0 * kb, na * kb);
+        if (vb) std::memcpy(oth_v + a0 * vb, cur_v + a0 * vb, na * vb);
+        continue;
+      }
+      const gs_status st = host_merge_runs(h, kdt, vdt, cur_k, cur_v, oth_k, oth_v, a0, na, nb, chunk, descending);
+      if (st != GS_OK) return st;
+    }
+    std::swap(cur_k, oth_k);
+    std::swap(cur_v, oth_v);
+  }
+  if (cur_k != user_k) {  // not reached: the chunks start where the level count puts the result
+    std::memcpy(user_k, cur_k, n * kb);
+    if (vb) std::memcpy(user_v, cur_v, n * vb);
+  }
+  return GS_OK;
+}
+
 }  // namespace gs
 
 using namespace gs;
@@ -236,6 +421,12 @@ gs_status gs_set_tile_order(gs_handle* h, int order) {
   return GS_OK;
 }
 
+gs_status gs_set_host_sort_budget(gs_handle* h, uint64_t bytes) {
+  GS_CHECK_HANDLE(h);
+  h->host_budget = bytes;
+  return GS_OK;
+}
+
 gs_status gs_synchronize(gs_handle* h) {
   GS_CHECK_HANDLE(h);
   DeviceScope ds(h->device);
@@ -281,6 +472,9 @@ gs_status gs_sort_keys_host(gs_handle* h, gs_dtype kdt, void* h_keys, uint64_t n
   if (n == 0) return GS_OK;
   if (!h_keys) return fail(GS_INVALID_ARG, "null key pointer");
   DeviceScope ds(h->device);
+  const uint64_t chunk = host_chunk_elems(h, dtype_bytes(kdt), 0);
+  if (chunk == 0) return fail(GS_OUT_OF_MEMORY, "Couldn't allocate device vector");
+  if (chunk < n) return host_sort_chunked(h, kdt, h_keys, GS_NONE, nullptr, n, descending, chunk);
   const size_t kbytes = n * (size_t)dtype_bytes(kdt);
   void* dk = nullptr;
   hipError_t e = h->arena.get(Arena::kStageKeys, kbytes, h->stream, &dk);
@@ -307,6 +501,13 @@ gs_status gs_sort_pairs_host(gs_handle* h, gs_dtype kdt, void* h_keys, gs_dtype 
     return fail(GS_INVALID_ARG, "pair layout (stride %u, value offset %u) cannot hold the key and value",
                 pair_stride, val_offset);
   DeviceScope ds(h->device);
+  const uint64_t chunk = host_chunk_elems(h, kb, vb);
+  if (chunk == 0) return fail(GS_OUT_OF_MEMORY, "Couldn't allocate device vector");
+  if (chunk < n) {
+    gs_status st = host_sort_chunked(h, kdt, h_keys, vdt, h_vals, n, descending, chunk);
+    if (st != GS_OK || !h_pairs_out) return st;
+    return host_pack_pairs(h, kb, h_keys, vb, h_vals, n, h_pairs_out, pair_stride, val_offset, chunk);
+  }
   void *dk = nullptr, *dv = nullptr;
   if (h->arena.get(Arena::kStageKeys, n * (size_t)kb, h->stream, &dk) != hipSuccess ||
       h->arena.get(Arena::kStageVals, n * (size_t)vb, h->stream, &dv) != hipSuccess)

@@ -100,6 +100,9 @@ struct gs_handle {
   // Tile hand-out of the handle's onesweep passes (gs::TilePolicy); the
   // server grants grouped hand-out to one slot at a time.
   int tile_policy = gs::kTilesByStream;
+  // Device bytes one host-memory sort may stage (gs_set_host_sort_budget);
+  // 0: the device's free memory at the call.
+  uint64_t host_budget = 0;
   gs::Arena arena;
   gs::Profiler profiler;
   gs::LaunchTimer* timer() { return profiler.active ? &profiler : nullptr; }

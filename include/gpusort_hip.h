@@ -135,10 +135,24 @@ gs_status gs_sort_pairs_noalloc(gs_handle* h, gs_dtype kdt, void* d_keys, gs_dty
  * and values are sorted in place and, if h_pairs_out is not NULL, the pairs
  * are also written there as records of pair_stride bytes with the key at
  * offset 0 and the value at val_offset (the std::pair<K,V> layout of
- * src/cuda_utils.cu:498-502). */
+ * src/cuda_utils.cu:498-502).
+ * Inputs of more than budget / (3 * (key + value bytes)) elements are sorted
+ * in chunks of that many, each on the device, then merged by a binary tree
+ * of stable 2-way merges streamed through the device in blocks of a chunk
+ * (the reference's SplitChunks and ParallelMergeArrays,
+ * src/cuda_utils.cu:508-520, :323-371); the budget is the device's free
+ * memory at the call unless gs_set_host_sort_budget set one.  A chunked
+ * call also allocates host memory of the input's size. */
 gs_status gs_sort_keys_host(gs_handle* h, gs_dtype kdt, void* h_keys, uint64_t n, int descending);
 gs_status gs_sort_pairs_host(gs_handle* h, gs_dtype kdt, void* h_keys, gs_dtype vdt, void* h_vals, uint64_t n,
                              int descending, void* h_pairs_out, uint32_t pair_stride, uint32_t val_offset);
+
+/* Device bytes one host-memory sort of this handle may stage; 0 (the
+ * default) means the device's free memory at each call, as the reference's
+ * SplitChunks reads it (src/cuda_utils.cu:508-520).  A smaller budget
+ * leaves room for other work on the device; a budget below three elements
+ * makes host-memory sorts fail with GS_OUT_OF_MEMORY. */
+gs_status gs_set_host_sort_budget(gs_handle* h, uint64_t bytes);
 
 /* Batched segmented sort: segment s is [d_offsets[s], d_offsets[s+1]) of the
  * device arrays (offsets in device memory, nseg+1 entries, non-decreasing).
