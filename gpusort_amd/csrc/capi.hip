@@ -9,7 +9,9 @@
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
+#include <memory>
 #include <mutex>
+#include <new>
 #include <string>
 #include <vector>
 
@@ -168,15 +170,24 @@ gs_status sort_device(gs_handle* h, gs_dtype kdt, void* d_keys, gs_dtype vdt, vo
 // ordered key images on the host, the two input slices go in, the device
 // merge (ties from the earlier run) writes the block, and it comes back.
 
-// Elements one device-staged piece may hold; 0 when not even one fits.
-static uint64_t host_chunk_elems(gs_handle* h, int kb, int vb) {
+// Staging of at most this many bytes (3 * n * element bytes) takes one piece
+// without asking the device for its free memory (the query costs about as
+// much as a small sort's kernels); such a call that does not fit fails with
+// GS_OUT_OF_MEMORY instead of being chunked.
+constexpr uint64_t kHostQueryBytes = 256ull << 20;
+
+// Elements one device-staged piece may hold for an n-element call; 0 when
+// not even one fits.
+static uint64_t host_chunk_elems(gs_handle* h, int kb, int vb, uint64_t n) {
+  const uint64_t per = 3ull * (uint64_t)(kb + vb);
   uint64_t budget = h->host_budget;
   if (budget == 0) {
+    if (n <= kHostQueryBytes / per) return n;
     size_t free_b = 0, total_b = 0;
     if (hipMemGetInfo(&free_b, &total_b) != hipSuccess) return 0;
     budget = free_b;
   }
-  return budget / (3ull * (uint64_t)(kb + vb));
+  return budget / per;
 }
 
 // Number of elements of a (na) among the first d of the stable merge of a
@@ -282,21 +293,19 @@ static gs_status host_sort_chunked(gs_handle* h, gs_dtype kdt, void* h_keys, gs_
   const uint64_t nchunks = (n + chunk - 1) / chunk;
   int height = 0;
   while ((1ull << height) < nchunks) ++height;
-  std::vector<unsigned char> tmp_k, tmp_v;
-  try {
-    tmp_k.resize(n * kb);
-    if (vb) tmp_v.resize(n * vb);
-  } catch (...) {
-    return fail(GS_OUT_OF_MEMORY, "host allocation failed");
-  }
+  // Host buffer of the input's size, not zero-filled (every byte is written
+  // before it is read).
+  std::unique_ptr<unsigned char[]> tmp_k(new (std::nothrow) unsigned char[n * kb]);
+  std::unique_ptr<unsigned char[]> tmp_v(vb ? new (std::nothrow) unsigned char[n * vb] : nullptr);
+  if (!tmp_k || (vb && !tmp_v)) return fail(GS_OUT_OF_MEMORY, "host allocation failed");
   unsigned char* user_k = static_cast<unsigned char*>(h_keys);
   unsigned char* user_v = static_cast<unsigned char*>(h_vals);
   // The sorted chunks go where an even number of merge levels leaves the
   // result in the caller's arrays.
-  unsigned char* cur_k = (height % 2) ? tmp_k.data() : user_k;
-  unsigned char* cur_v = (height % 2) ? tmp_v.data() : user_v;
-  unsigned char* oth_k = (height % 2) ? user_k : tmp_k.data();
-  unsigned char* oth_v = (height % 2) ? user_v : tmp_v.data();
+  unsigned char* cur_k = (height % 2) ? tmp_k.get() : user_k;
+  unsigned char* cur_v = (height % 2) ? tmp_v.get() : user_v;
+  unsigned char* oth_k = (height % 2) ? user_k : tmp_k.get();
+  unsigned char* oth_v = (height % 2) ? user_v : tmp_v.get();
   void *dk = nullptr, *dv = nullptr;
   if (h->arena.get(Arena::kStageKeys, chunk * kb, h->stream, &dk) != hipSuccess ||
       (vb && h->arena.get(Arena::kStageVals, chunk * vb, h->stream, &dv) != hipSuccess))
@@ -472,7 +481,7 @@ gs_status gs_sort_keys_host(gs_handle* h, gs_dtype kdt, void* h_keys, uint64_t n
   if (n == 0) return GS_OK;
   if (!h_keys) return fail(GS_INVALID_ARG, "null key pointer");
   DeviceScope ds(h->device);
-  const uint64_t chunk = host_chunk_elems(h, dtype_bytes(kdt), 0);
+  const uint64_t chunk = host_chunk_elems(h, dtype_bytes(kdt), 0, n);
   if (chunk == 0) return fail(GS_OUT_OF_MEMORY, "Couldn't allocate device vector");
   if (chunk < n) return host_sort_chunked(h, kdt, h_keys, GS_NONE, nullptr, n, descending, chunk);
   const size_t kbytes = n * (size_t)dtype_bytes(kdt);
@@ -501,7 +510,7 @@ gs_status gs_sort_pairs_host(gs_handle* h, gs_dtype kdt, void* h_keys, gs_dtype 
     return fail(GS_INVALID_ARG, "pair layout (stride %u, value offset %u) cannot hold the key and value",
                 pair_stride, val_offset);
   DeviceScope ds(h->device);
-  const uint64_t chunk = host_chunk_elems(h, kb, vb);
+  const uint64_t chunk = host_chunk_elems(h, kb, vb, n);
   if (chunk == 0) return fail(GS_OUT_OF_MEMORY, "Couldn't allocate device vector");
   if (chunk < n) {
     gs_status st = host_sort_chunked(h, kdt, h_keys, vdt, h_vals, n, descending, chunk);

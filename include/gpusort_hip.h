@@ -141,8 +141,10 @@ gs_status gs_sort_pairs_noalloc(gs_handle* h, gs_dtype kdt, void* d_keys, gs_dty
  * of stable 2-way merges streamed through the device in blocks of a chunk
  * (the reference's SplitChunks and ParallelMergeArrays,
  * src/cuda_utils.cu:508-520, :323-371); the budget is the device's free
- * memory at the call unless gs_set_host_sort_budget set one.  A chunked
- * call also allocates host memory of the input's size. */
+ * memory at the call unless gs_set_host_sort_budget set one (an input
+ * whose staging, 3 * n * element bytes, is at most 256 MiB skips that query
+ * and is sorted in one piece).  A chunked call also allocates host memory of
+ * the input's size. */
 gs_status gs_sort_keys_host(gs_handle* h, gs_dtype kdt, void* h_keys, uint64_t n, int descending);
 gs_status gs_sort_pairs_host(gs_handle* h, gs_dtype kdt, void* h_keys, gs_dtype vdt, void* h_vals, uint64_t n,
                              int descending, void* h_pairs_out, uint32_t pair_stride, uint32_t val_offset);
