@@ -133,3 +133,19 @@ def test_bucket_table_edges(handle, kdt, upper):
         say(f"bucket table edge {name} kdt={np.dtype(kdt).name} upper={upper} nspl={spl.size}")
         check(handle, keys, np.uint32, spl, upper, descending=False)
         check(handle, keys, None, spl[::-1].copy(), upper, descending=True)
+
+
+@pytest.mark.parametrize("kdt", [np.uint32, np.int64], ids=lambda d: np.dtype(d).name)
+@pytest.mark.parametrize("span", [2000, 6000, 12000, 30000])
+@pytest.mark.parametrize("upper", [False, True], ids=["lower_cut", "upper_cut"])
+def test_small_spans(handle, kdt, span, upper):
+    """Dense keys around splitters whose span maps one to a few key images to
+    a table entry (the counting read's refined entries then cover one or two
+    images): every key value of the span and its surroundings occurs."""
+    n = 200_001
+    base = 1_000_000
+    keys = (base - 300 + (datagen.draws(span, n) % np.uint64(span + 600)).astype(np.int64)).astype(kdt)
+    spl = np.sort(base + (datagen.draws(span + 1, 40) % np.uint64(span + 1)).astype(np.int64)).astype(kdt)
+    say(f"multisplit small span {span} kdt={np.dtype(kdt).name} upper={upper}")
+    check(handle, keys, np.uint32, spl, upper, descending=False)
+    check(handle, keys, None, spl[::-1].copy(), upper, descending=True)
