@@ -824,7 +824,7 @@ def main() -> int:
     args = ap.parse_args()
 
     if args.gpus != 1:
-        print(f"bench.py measures one GPU: --gpus {args.gpus} is not supported in this build "
+        print(f"bench.py measures one GPU: --gpus {args.gpus} is not supported "
               f"(the per-rank path runs p virtual ranks on one device).", file=sys.stderr)
         return 2
 

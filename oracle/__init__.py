@@ -4,7 +4,7 @@ ctypes binding of ``oracle/liboracle.so`` (the CPU restatement of the
 reference's algorithms, ``oracle.cpp``) and, when built, of
 ``oracle/_ref/libref_omp.so`` (the reference's own OmpUtils compiled from its
 source tree).  Only ``tests/``, ``__graft_entry__.smoke()`` and the
-cpu_baseline leg of ``bench.py`` use it, always as the checker or the
+cpu_baseline leg of ``bench.py`` use it, always to check results or as the
 reported CPU baseline, never as the measured GPU path.
 
 Every function follows a file:line of monotone-RK/gpusort cited in

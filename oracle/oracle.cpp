@@ -25,8 +25,8 @@
 // 215-226), which nothing but the reference's own run could reproduce.  The
 // vrank_* functions restate the round structure with a deterministic sample
 // rule instead (regular samples, the same rule as the GPU path): the rule is
-// this build's, the structure (groups, partners, lower/upper bound buckets,
-// merge of received runs, rebalance) is the reference's.
+// the GPU path's own, the structure (groups, partners, lower/upper bound
+// buckets, merge of received runs, rebalance) is the reference's.
 #include <omp.h>
 #include <stdint.h>
 #include <string.h>

@@ -1,4 +1,4 @@
-"""Pins the CPU oracle (oracle/oracle.cpp) before it is trusted as the checker.
+"""Pins the CPU oracle (oracle/oracle.cpp) before its results are trusted in the parity tests.
 
 * Golden vectors of the reference's own OmpUtils (tests/golden/, produced by
   make_golden.py from include/omp/omp_utils.h compiled out of the reference
